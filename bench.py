@@ -1,0 +1,170 @@
+"""Benchmark: particle-steps/s of the MPS time step on one MI355X.
+
+Workload: 3D dam break, plain MPS, FP64 on the GPU — l0 = 0.005 m, water
+100 x 100 x 80 particles in a 1.6 x 0.6 x 0.405 m open tank with one wall and
+two dummy layers (1,175,450 particles), radii 2.1 l0 (PND / gradient) and
+4.0 l0 (Laplacian), dt = 5e-5 s, Jacobi-PCG to relative residual 1e-8, the
+neighbour lists rebuilt every step. A step is one full pass of the time loop
+(PND, neighbour grid, viscosity, predictor, PND*, bcon, PPE assembly + solve,
+p-hat, pressure gradient, integration, collision, prepdata) with the state
+resident in HBM.
+
+Prints one JSON line. The roofline entry is for the PPE SpMV kernel (the CG
+K1 pass): algorithmic bytes per launch / mean launch time from HIP events
+recorded on the library's stream during the timed steps.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(nu, nnz_total):
+    """Compulsory bytes of one q = A p launch: off-diagonal values (8 B) and
+    column ids (4 B), diagonal (8 B), row length (4 B), p read once (8 B) and
+    q written (8 B) per row; gathers of p are counted once."""
+    off = nnz_total - nu
+    return off * 12 + nu * (8 + 4 + 8 + 8)
+
+
+def cpu_baseline(sample_scale=2, steps=2):
+    """Time the CPU oracle (single thread) on a bounded sample of the same
+    workload: the 3D dam break at 1/sample_scale linear scale."""
+    import voxarmps_amd as vx
+    from oracle.pyoracle import oracle_backend
+
+    case = vx.dam_break_3d_c1(scale=sample_scale)
+    p = vx.dam3d_params(case)
+    s = vx.Simulation(p, case, backend=oracle_backend())
+    s.init()
+    t0 = time.perf_counter()
+    s.step(steps)
+    el = time.perf_counter() - t0
+    s.close()
+    return {
+        "value": case.tp * steps / el,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} time steps of the 3D dam break at 1/{sample_scale} linear scale "
+                  f"({case.tp} particles), sequential C oracle, 1 thread",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=1, help="linear down-scaling of the workload")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import voxarmps_amd as vx
+
+    be = vx.hip_backend()
+    nd = C.c_int(0)
+    be.lib.vox_device_count(C.byref(nd))
+    if nd.value < 1:
+        raise SystemExit("no HIP device visible")
+
+    case = vx.dam_break_3d_c1(scale=args.scale)
+    p = vx.dam3d_params(case)
+    sim = vx.Simulation(p, case, backend=be)
+    sim.init()
+    print(f"# init done: {case.tp} particles", file=sys.stderr, flush=True)
+    if args.warmup:
+        sim.step(args.warmup)
+    be.lib.vox_sync(sim.ctx)
+    print("# warmup done", file=sys.stderr, flush=True)
+
+    be.lib.vox_prof_reset(sim.ctx)
+    be.lib.vox_prof_enable(sim.ctx, 1)
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step(1)
+        iters.append(sim.status().last_iters)
+    be.lib.vox_sync(sim.ctx)
+    el = time.perf_counter() - t0
+    be.lib.vox_prof_enable(sim.ctx, 0)
+    print("# timed steps done", file=sys.stderr, flush=True)
+
+    st = sim.status()
+    info = sim.ppe()
+    nu = int(info["rowptr"].shape[0] - 1)
+    nnz = int(info["rowptr"][-1])
+    ms = C.c_double(0.0)
+    nl = C.c_int64(0)
+    be.lib.vox_prof_get(sim.ctx, b"spmv", C.byref(ms), C.byref(nl))
+    avg_ms = ms.value / max(nl.value, 1)
+    bytes_launch = spmv_bytes(nu, nnz)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    per_kernel = {}
+    for k in ("list", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p",
+              "collision", "pgrad", "phat", "prep", "euler", "bcon", "visc", "vfrac", "misc"):
+        t = C.c_double(0.0)
+        n = C.c_int64(0)
+        be.lib.vox_prof_get(sim.ctx, k.encode(), C.byref(t), C.byref(n))
+        per_kernel[k] = round(t.value / max(args.steps, 1), 4)
+    sim.close()
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    value = case.tp * args.steps / el
+    out = {
+        "metric": "particle-steps/s at 3D 1M",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D dam break, plain MPS, FP64, l0=0.005 m, water 100x100x80 in "
+                        "1.6x0.6x0.405 m tank, 1 wall + 2 dummy layers",
+            "particles": case.tp,
+            "fluid": case.fp,
+            "re_s": "2.1 l0", "re_l": "4.0 l0", "dt": p.dt,
+            "ppe": f"Jacobi-PCG FP64 to rel-res {p.cg_tol:g}",
+            "cg_iters_mean": sum(iters) / len(iters),
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "kernel": "k_cg_spmv (PPE q = A p + p.q)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg_ms,
+            "launches": nl.value,
+            "unknowns": nu,
+            "nnz": nnz,
+        },
+        "stage_ms_per_step": per_kernel,
+        "cpu_baseline": cpu,
+        "status": {"collisions": st.collisions, "nan": st.nan_count, "out_of_box": st.out_of_box,
+                   "max_count_s": st.max_count_s, "max_count_l": st.max_count_l},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

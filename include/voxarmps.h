@@ -1,0 +1,265 @@
+/*
+ * voxarmps.h — C ABI of the MI355X-native MPS time-step pipeline.
+ *
+ * This boundary replaces the per-stage host routines of the reference
+ * (VoxarMPS/VoxarMPS/functions.cuh:56-111, neighbour_parallel.h:6-7) and the
+ * kernel-launch branch of its time loop (VoxarMPS/VoxarMPS/main.cu:1027-1363).
+ * The reference routines take 1-based host arrays of length TP+1 and a
+ * std::string mode argument; here the state lives in device memory owned by a
+ * context, host arrays are 0-based with TP elements (a reference caller passes
+ * `x + 1`), and every entry point returns a status code (VOX_OK == 0) with a
+ * message available from vox_last_error().
+ *
+ * Plain C types only: no torch, no HIP types in any signature.
+ */
+#ifndef VOXARMPS_H
+#define VOXARMPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define VOX_OK               0
+#define VOX_ERR_ARG          1  /* invalid argument or unsupported option      */
+#define VOX_ERR_HIP          2  /* HIP runtime failure (allocation, launch)    */
+#define VOX_ERR_OVERFLOW     3  /* neighbour row exceeded its capacity         */
+#define VOX_ERR_DOMAIN       4  /* particle outside the neighbour grid box     */
+#define VOX_ERR_SOLVER       5  /* PPE solver breakdown / no convergence       */
+#define VOX_ERR_NOLIB        6  /* requested path not built                    */
+#define VOX_ERR_STATE        7  /* call order violated (e.g. step before init) */
+
+/* ---- field identifiers (names follow main.cu:211-232, 554-770) ---------- */
+enum vox_field {
+    VOX_F_X = 0, VOX_F_Y, VOX_F_Z,            /* positions                  */
+    VOX_F_U, VOX_F_V, VOX_F_W,                /* velocities                 */
+    VOX_F_P,                                  /* pressure                   */
+    VOX_F_PTYPE,                              /* int32: -1 dummy, 0 wall, 1/2 fluid */
+    VOX_F_XSTAR, VOX_F_YSTAR, VOX_F_ZSTAR,    /* predicted positions        */
+    VOX_F_USTAR, VOX_F_VSTAR, VOX_F_WSTAR,    /* predicted velocities       */
+    VOX_F_UNEW, VOX_F_VNEW, VOX_F_WNEW,       /* corrected velocities       */
+    VOX_F_PNEW,                               /* solved pressure            */
+    VOX_F_PHAT,                               /* minimum neighbour pressure */
+    VOX_F_N, VOX_F_NSTAR,                     /* PND at x and at x*         */
+    VOX_F_NEUT,                               /* SPS eddy viscosity         */
+    VOX_F_TURB1, VOX_F_TURB2, VOX_F_TURB3,    /* SPS stress divergence      */
+    VOX_F_C,                                  /* volume fraction of fluid 2 */
+    VOX_F_MEU,                                /* dynamic viscosity          */
+    VOX_F_RHO,                                /* density                    */
+    VOX_F_BCON,                               /* int32: -1 / 0 / 1          */
+    VOX_F_SOURCE,                             /* PPE right-hand side        */
+    VOX_F_COUNT
+};
+
+/* neighbour list selector */
+#define VOX_LIST_S 0   /* radius re_s: PND, gradient, SPS, collision, p-hat */
+#define VOX_LIST_L 1   /* radius re_l: Laplacian (viscous term, PPE matrix)  */
+
+/* precision modes */
+#define VOX_PREC_FP64  0   /* every kernel in FP64                           */
+#define VOX_PREC_MIXED 1   /* FP32 particle kernels + SpMV, FP64 dots + IR   */
+
+/* PPE solvers */
+#define VOX_SOLVER_JACOBI_CG 0
+
+/* boundary-clamp variants of BOUNDARIE (functions.cu:1963-2002) */
+#define VOX_TEST_DAM  0
+#define VOX_TEST_TURB 1
+#define VOX_TEST_DROP 2
+
+/*
+ * Every constant main.cu hard-codes (main.cu:211-494), plus the two-radius,
+ * time-step, solver and precision extensions. Fill with vox_params_default()
+ * and override. Only int32 and double members, in this order (ctypes mirrors it).
+ */
+typedef struct vox_params {
+    int32_t dim;          /* 2 or 3 (main.cu:309-317)                            */
+    int32_t ktype;        /* weight function: 2 rational, 6 cubic (main.cu:288-292) */
+    double  dl;           /* particle spacing DL (main.cu:248-250)               */
+    double  re_s;         /* radius of PND / gradient stages                    */
+    double  re_l;         /* radius of Laplacian stages (== re_s: reference)     */
+    double  delta;        /* grid expansion DELTA = DL/5 (main.cu:282)           */
+    double  lambda;       /* Laplacian coefficient (main.cu:491-494)             */
+    double  rho1, nu1, rho2, nu2;             /* fluid properties (main.cu:244-247) */
+    double  gx, gy, gz;                       /* gravity (main.cu:321-330)       */
+    double  dt;           /* time step                                           */
+    int32_t dt_mode;      /* 0 fixed dt, 1 DTcalculation (functions.cu:1684-1698) */
+    int32_t turb;         /* SPS-LES on (main.cu:294-303)                        */
+    double  courant, c01, c02, dt_max;        /* DTcalculation inputs             */
+    double  dirichlet;    /* free-surface threshold on nstar/n0 (main.cu:228)   */
+    double  relaxp;       /* pressure relaxation (main.cu:263-265)               */
+    double  coll;         /* collision distance (main.cu:269-273)                */
+    double  cc;           /* collision coefficient (main.cu:277-279)             */
+    double  vel_cor;      /* BOUNDARIE velocity factor (main.cu:275, 506)        */
+    double  cs;           /* Smagorinsky constant (main.cu:281)                  */
+    double  ncorrection;  /* PND correction for ktype != 2 (main.cu:284)         */
+    double  vmax;         /* damper speed (main.cu:331)                          */
+    double  pmin, pmax;   /* pressure clamp (main.cu:351, 548)                   */
+    int32_t fluid2_type;  /* 0 Newtonian, 1 Herschel-Bulkley (main.cu:357-377)   */
+    int32_t kh_correction;/* 1: symmetric p-hat gradient (main.cu:344)           */
+    double  n_pl, meu0, phi, cohes;           /* rheology (main.cu:358-372)       */
+    double  xmin, xmax, ymin, ymax, zmin, zmax; /* grid box (main.cu:692-715)    */
+    int32_t test_kind;    /* VOX_TEST_* (BOUNDARIE / PREPDATA variants)          */
+    int32_t coef2d;       /* 1: keep the 2D constants 4 and 2 of PREDICTION /
+                             PRESSGRAD in 3D as the reference does              */
+    int32_t precision;    /* VOX_PREC_*                                          */
+    int32_t solver;       /* VOX_SOLVER_*                                        */
+    double  cg_tol;       /* relative residual target ||r||/||b||               */
+    int32_t cg_maxit;     /* iteration cap                                       */
+    int32_t cg_check;     /* iterations between host convergence checks         */
+    int32_t cap_s;        /* neighbour row capacity for list S (0 = auto)        */
+    int32_t cap_l;        /* neighbour row capacity for list L (0 = auto)        */
+} vox_params;
+
+/* PPE solve report */
+typedef struct vox_solve_info {
+    int32_t iterations;   /* CG iterations taken                                 */
+    int32_t converged;    /* 1 if ||r||/||b|| <= tol                              */
+    int32_t unknowns;     /* rows with bcon == 0                                 */
+    int32_t nnz;          /* stored off-diagonal + diagonal entries              */
+    double  relres;       /* final ||r||/||b|| (FP64)                            */
+    double  bnorm;        /* ||b||                                               */
+} vox_solve_info;
+
+/* per-step status counters (PREPDATA checks, functions.cu:2718-2743) */
+typedef struct vox_status {
+    int32_t nan_count;        /* ERROR#1 particles                               */
+    int32_t out_of_box;       /* ERROR#2 particles                               */
+    int32_t max_count_s;      /* longest row of list S                           */
+    int32_t max_count_l;      /* longest row of list L                           */
+    int32_t collisions;       /* applied collision impulses in the last step     */
+    int32_t collision_cand;   /* candidate pairs screened by the exact pass      */
+    int32_t steps;            /* steps completed since vox_init                  */
+    int32_t last_iters;       /* CG iterations of the last solve                 */
+    double  last_relres;      /* residual of the last solve                      */
+    double  n0_s, n0_l;       /* reference densities                             */
+    double  time;             /* simulated time                                  */
+} vox_status;
+
+typedef struct vox_ctx vox_ctx;
+
+/* ---- library / context --------------------------------------------------- */
+const char* vox_last_error(void);
+const char* vox_version(void);
+/* sizeof of the three ABI structs, so a binding can check its mirror. */
+int vox_abi_sizes(int32_t* params, int32_t* solve_info, int32_t* status);
+int vox_device_count(int* count);
+
+/* Reference defaults for a fully incompressible run (main.cu:211-494). */
+int vox_params_default(vox_params* p, int dim, double dl);
+/* Grid box from positions: min/max over particles expanded by DL (main.cu:692-715). */
+int vox_params_box(vox_params* p, int tp, const double* x, const double* y, const double* z);
+
+/* Allocate device state for tp particles ordered ghost -> wall -> fluid
+ * (gp, wp, fp counts; main.cu:635-682). */
+int vox_create(vox_ctx** ctx, const vox_params* p, int tp, int gp, int wp, int fp);
+int vox_destroy(vox_ctx* ctx);
+int vox_get_params(vox_ctx* ctx, vox_params* p);
+/* Change run-time parameters (dt, solver tolerances, radii unchanged). */
+int vox_set_params(vox_ctx* ctx, const vox_params* p);
+
+/* Host <-> device copies of one field, 0-based, tp elements (int32 for PTYPE/BCON). */
+int vox_upload(vox_ctx* ctx, int field, const void* host);
+int vox_download(vox_ctx* ctx, int field, void* host);
+/* Upload the initial state read from a .grd file (x, y, z may be NULL in 2D for z). */
+int vox_upload_state(vox_ctx* ctx, const double* x, const double* y, const double* z,
+                     const double* u, const double* v, const double* w,
+                     const double* p, const int32_t* ptype);
+
+/* ---- stage entry points (one per reference host routine) ----------------- */
+/* Density by particle type (main.cu:966-978). */
+int vox_density(vox_ctx* ctx);
+/* NEIGHBOR (functions.cuh:74, functions.cu:2788-2929): rebuild lists S and L. */
+int vox_neighbor(vox_ctx* ctx);
+/* PNUM (functions.cuh:56, functions.cu:94-123). loopstarted == 0 also copies
+ * p -> pnew and sets n0 = max n (and n0_l on list L). */
+int vox_pnum(vox_ctx* ctx, int loopstarted);
+/* PNUMSTAR (functions.cuh:58, functions.cu:157-180). */
+int vox_pnumstar(vox_ctx* ctx);
+/* SPS (functions.cuh:80, functions.cu:1719-1841). */
+int vox_sps(vox_ctx* ctx);
+/* V_FRACTION, method 2 (functions.cuh:105, functions.cu:2580-2638). */
+int vox_vfraction(vox_ctx* ctx);
+/* VISCOSITY (functions.cuh:88, functions.cu:2053-2184). */
+int vox_viscosity(vox_ctx* ctx);
+/* PREDICTION (functions.cuh:91, functions.cu:2283-2369). */
+int vox_prediction(vox_ctx* ctx);
+/* BCON (functions.cuh:97, functions.cu:433-445). */
+int vox_bcon(vox_ctx* ctx);
+/* MATRIX, Method 1 without matopt/srcopt (functions.cuh:62, functions.cu:869-1005):
+ * device CSR over the bcon == 0 rows plus right-hand side. */
+int vox_ppe_assemble(vox_ctx* ctx);
+/* Solve the assembled PPE into pnew from a zero start (replaces INCDECOM + CGM,
+ * functions.cu:1139-1200, 506-716). */
+int vox_ppe_solve(vox_ctx* ctx, vox_solve_info* info);
+/* PRESSURECALC, Method 1 (functions.cuh:101, functions.cu:2465-2549):
+ * pnew = 0, assemble, solve, clamp to [pmin, pmax] with NaN -> pmax. */
+int vox_pressurecalc(vox_ctx* ctx, vox_solve_info* info);
+/* PHATCALC (functions.cuh:69, functions.cu:207-224). */
+int vox_phat(vox_ctx* ctx);
+/* PRESSGRAD (functions.cuh:71, functions.cu:245-355). */
+int vox_pressgrad(vox_ctx* ctx);
+/* EULERINTEGRATION + BOUNDARIE (functions.cuh:85, functions.cu:1963-2021). */
+int vox_euler(vox_ctx* ctx);
+/* COLLISION2 (functions.cuh:76, functions.cu:1523-1601), with the reference's
+ * in-place sequential semantics reproduced exactly. */
+int vox_collision(vox_ctx* ctx);
+/* PREPDATA (functions.cuh:108, functions.cu:2688-2746). */
+int vox_prepdata(vox_ctx* ctx);
+/* DTcalculation (functions.cuh:78, functions.cu:1684-1698) when dt_mode == 1. */
+int vox_dtcalc(vox_ctx* ctx);
+
+/* Initialisation of main.cu:966-1012: density, NEIGHBOR, PNUM(loopstarted=0). */
+int vox_init(vox_ctx* ctx);
+/* nsteps iterations of the time loop body of main.cu:1027-1363 (no I/O). */
+int vox_step(vox_ctx* ctx, int nsteps);
+int vox_sync(vox_ctx* ctx);
+int vox_get_status(vox_ctx* ctx, vox_status* st);
+/* Reference densities; vox_set_n0 lets a caller impose its own. */
+int vox_get_n0(vox_ctx* ctx, double* n0_s, double* n0_l);
+int vox_set_n0(vox_ctx* ctx, double n0_s, double n0_l);
+
+/* ---- neighbour lists ----------------------------------------------------- */
+/* Row lengths (tp ints) and row-major ids (0-based, row i at ids[i*stride]). */
+int vox_get_neighbors(vox_ctx* ctx, int which, int32_t* counts, int32_t* ids, int stride);
+/* The reference slot layout (functions.cu:2919-2920): a flat (tp+1)*stride
+ * buffer, neighb[I*stride+1] = count+1, neighb[I*stride+2..] = 1-based ids. */
+int vox_get_neighbors_slot(vox_ctx* ctx, int which, int32_t* neighb, int stride);
+/* Cell key per particle (0-based cell number of functions.cu:2822-2834). */
+int vox_get_cells(vox_ctx* ctx, int which, int32_t* cell);
+
+/* ---- PPE export (dumps for training / checks) ---------------------------- */
+/* First call with NULL arrays to get nu and nnz; then pass arrays of that size:
+ * rowptr[nu+1], cols[nnz] (unknown indices), vals[nnz], rhs[nu], ids[nu]
+ * (particle id of each unknown). */
+int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
+                   int32_t* cols, double* vals, double* rhs, int32_t* ids);
+
+/* ---- profiling ----------------------------------------------------------- */
+/* When enabled, HIP events bracket every launch of the named kernel classes on
+ * the context's stream; vox_prof_get returns total ms and launch count. */
+int vox_prof_enable(vox_ctx* ctx, int on);
+int vox_prof_get(vox_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+int vox_prof_reset(vox_ctx* ctx);
+
+/* ---- .grd input (main.cu:496-682) and VTU output (inOut.cpp:104-336) ------ */
+/* Read the particle count of a .grd file. */
+int vox_grd_count(const char* path, int* tp);
+/* Read a .grd file into caller arrays of tp elements (z, w ignored in 2D). */
+int vox_grd_read(const char* path, int dim, int tp, double* x, double* y, double* z,
+                 double* u, double* v, double* w, double* p, int32_t* ptype,
+                 int* gp, int* wp, int* fp);
+/* Write a .grd file with full precision (%.17g). */
+int vox_grd_write(const char* path, int dim, int tp, const double* x, const double* y,
+                  const double* z, const double* u, const double* v, const double* w,
+                  const double* p, const int32_t* ptype);
+/* Write an ASCII UnstructuredGrid VTU of all particles (3D writes z and w,
+ * which the reference omits: inOut.cpp:132-136). fluid_only writes PTYPE > 0. */
+int vox_write_vtu(vox_ctx* ctx, const char* path, int fluid_only);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXARMPS_H */

@@ -1,0 +1,134 @@
+"""CPU tests of the case generator, the .grd format and the C ABI surface
+(library loads, exports every declared symbol, struct layouts agree, argument
+errors are reported) — no GPU work is launched."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import voxarmps_amd as vx
+from voxarmps_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_splitmix64_known_answers():
+    # first outputs of SplitMix64 seeded with 0 (Vigna's reference generator)
+    z = vx.cases.splitmix64(0, 3)
+    assert [int(v) for v in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_jitter_bounds_and_determinism():
+    c = vx.dam_break_2d_c0()
+    a = vx.jitter(c, 0.001, seed=0)
+    b = vx.jitter(c, 0.001, seed=0)
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    d = np.concatenate([a.x - c.x, a.y - c.y])
+    assert d.min() >= -0.001 and d.max() < 0.001 and abs(d.mean()) < 1e-4
+
+
+def test_dam_break_counts_and_order():
+    c0 = vx.dam_break_2d_c0()
+    assert c0.counts() == (2105, 574, 281, 1250)
+    assert np.all(np.diff(c0.ptype) >= 0)                # dummy -> wall -> fluid
+    c1 = vx.dam_break_3d_c1()
+    assert c1.fp == 800000 and c1.tp == 1175450
+    assert np.all(np.diff(c1.ptype) >= 0)
+    f = c1.ptype > 0
+    assert np.isclose(c1.x[f].min(), 0.005) and np.isclose(c1.x[f].max(), 0.5)
+    assert np.isclose(c1.z[f].max(), 0.4)
+    # no two particles share a lattice site
+    key = np.round(np.stack([c1.x, c1.y, c1.z], 1) / 0.005).astype(np.int64)
+    assert np.unique(key, axis=0).shape[0] == c1.tp
+
+
+def test_default_params_match_c_abi():
+    be = vx.hip_backend()
+    for dim, dl in ((2, 0.01), (3, 0.005)):
+        a = vx.VoxParams()
+        assert be.lib.vox_params_default(C.byref(a), dim, dl) == 0
+        b = vx.default_params(dim, dl)
+        assert a.as_dict() == b.as_dict()
+
+
+def test_params_box_matches_python():
+    be = vx.hip_backend()
+    c = vx.jitter(vx.dam_break(3, 0.005, (8, 8, 8), (20, 12, 9)), 0.0005)
+    a = vx.default_params(3, 0.005)
+    xd, yd, zd = (np.ascontiguousarray(v) for v in (c.x, c.y, c.z))
+    dp = C.POINTER(C.c_double)
+    assert be.lib.vox_params_box(C.byref(a), c.tp, xd.ctypes.data_as(dp), yd.ctypes.data_as(dp),
+                                 zd.ctypes.data_as(dp)) == 0
+    b = vx.set_box(vx.default_params(3, 0.005), c)
+    for k in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax"):
+        assert getattr(a, k) == getattr(b, k)
+
+
+def test_library_exports_every_declared_symbol():
+    be = vx.hip_backend()
+    hdr = open(os.path.join(ROOT, "include", "voxarmps.h")).read()
+    names = set(re.findall(r"\b(vox_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(be.lib, n)]
+    assert not missing, missing
+    assert set(_abi.PROTOTYPES) <= names | {"vox_last_error", "vox_version"}
+    _abi.check_sizes(be.lib)
+    assert b"gfx950" in be.lib.vox_version()
+
+
+def test_create_rejects_bad_arguments():
+    be = vx.hip_backend()
+    p = vx.default_params(2, 0.01)
+    ctx = C.c_void_p()
+    # counts that do not add up
+    rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 10, 1, 1, 1)
+    assert rc == _abi.VOX_ERR_ARG and b"gp + wp + fp" in be.lib.vox_last_error()
+    # empty grid box
+    rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 3, 1, 1, 1)
+    assert rc == _abi.VOX_ERR_ARG and b"box" in be.lib.vox_last_error()
+    # unsupported precision mode is refused, not silently run
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 1.0, 0.0, 1.0
+    p.precision = 7
+    rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 3, 1, 1, 1)
+    assert rc != 0
+
+
+def test_grd_round_trip_through_c_reader(tmp_path):
+    be = vx.hip_backend()
+    for c in (vx.jitter(vx.dam_break_2d_c0(), 0.001),
+              vx.jitter(vx.dam_break(3, 0.005, (6, 6, 6), (14, 10, 7)), 0.0005)):
+        c.u = np.linspace(-1, 1, c.tp)
+        c.p = np.linspace(0, 5000, c.tp)
+        path = str(tmp_path / f"case{c.dim}.grd")
+        vx.cases.write_grd(path, c)
+        n = C.c_int()
+        assert be.lib.vox_grd_count(path.encode(), C.byref(n)) == 0 and n.value == c.tp
+        arrs = [np.zeros(c.tp) for _ in range(7)]
+        pt = np.zeros(c.tp, np.int32)
+        g, w, f = C.c_int(), C.c_int(), C.c_int()
+        dp = C.POINTER(C.c_double)
+        rc = be.lib.vox_grd_read(path.encode(), c.dim, c.tp, *[a.ctypes.data_as(dp) for a in arrs],
+                                 pt.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(g), C.byref(w),
+                                 C.byref(f))
+        assert rc == 0
+        x, y, z, u, v, ww, p = arrs
+        assert np.array_equal(x, c.x) and np.array_equal(y, c.y) and np.array_equal(u, c.u)
+        assert np.array_equal(p, c.p) and np.array_equal(pt, c.ptype)
+        if c.dim == 3:
+            assert np.array_equal(z, c.z)
+        assert (g.value, w.value, f.value) == (c.gp, c.wp, c.fp)
+        back = vx.cases.read_grd(path, c.dim)
+        assert np.array_equal(back.x, c.x) and np.array_equal(back.ptype, c.ptype)
+
+
+def test_reference_input_files_parse(tmp_path):
+    """The reference's own .grd inputs hold 6 (2D) / 8 (3D) columns per row."""
+    ref = "/root/reference/inputs"
+    if not os.path.isdir(ref):
+        pytest.skip("reference inputs not present on this machine")
+    c = vx.cases.read_grd(os.path.join(ref, "input-dam.grd"), 2)
+    assert c.tp == 1122 and (c.gp, c.wp, c.fp) == (162, 312, 648)
+    c3 = vx.cases.read_grd(os.path.join(ref, "input-dam3d.grd"), 3)
+    assert c3.tp == 8140 and (c3.gp, c3.wp, c3.fp) == (2732, 4384, 1024)

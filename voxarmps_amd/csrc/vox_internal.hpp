@@ -1,0 +1,326 @@
+// vox_internal.hpp — device context and helpers shared by the HIP sources of
+// libvoxarmps. Everything here is gfx950 (wave64) code; no other target.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/voxarmps.h"
+
+namespace vox {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 waves)
+constexpr int kWave = 64;            // CDNA wavefront width
+constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
+
+// ---- error plumbing ----------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define VOX_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return ::vox::fail(VOX_ERR_HIP, std::string(#call) + ": " +            \
+                                                hipGetErrorString(e_));            \
+    } while (0)
+
+#define VOX_TRY(expr)                                                              \
+    do {                                                                           \
+        int rc_ = (expr);                                                          \
+        if (rc_ != VOX_OK) return rc_;                                             \
+    } while (0)
+
+inline int grid_for(long n, int block = kBlock)
+{
+    long g = (n + block - 1) / block;
+    return (int)(g < 1 ? 1 : g);
+}
+
+// ---- device flags (int32 slots of ctx->dflags) -------------------------------
+enum DevFlag {
+    FLAG_DOMAIN = 0,      // particle outside neighbour grid
+    FLAG_OVERFLOW_S,      // list S row longer than capacity
+    FLAG_OVERFLOW_L,
+    FLAG_MAXCNT_S,        // longest row (atomicMax)
+    FLAG_MAXCNT_L,
+    FLAG_NAN,             // PREPDATA ERROR#1 count
+    FLAG_OOB,             // PREPDATA ERROR#2 count
+    FLAG_COLL_HITS,       // applied collision impulses
+    FLAG_COLL_CAND,       // candidate pairs of the exact pass
+    FLAG_COLL_VIOLATION,  // margin of the candidate screen exceeded
+    FLAG_NU,              // number of PPE unknowns
+    FLAG_CG_DONE,         // CG finished (converged, breakdown or cap)
+    FLAG_CG_ITERS,
+    FLAG_CG_CONV,
+    FLAG_CG_BREAK,
+    FLAG_COUNTER0,        // last-block tickets for reductions (several)
+    FLAG_COUNTER1,
+    FLAG_COUNTER2,
+    FLAG_COUNT
+};
+
+// device scalar slots of ctx->dscal (double)
+enum DevScal {
+    SC_BB = 0, SC_BNORM, SC_RZ, SC_PQ, SC_ALPHA, SC_BETA, SC_RR, SC_RELRES, SC_NNZ,
+    SC_COUNT
+};
+
+// ---- neighbour grid + list ------------------------------------------------------
+struct Grid {
+    double xmin, ymin, zmin;
+    double hx, hz;          // cell width in x/y and in z (re and re+DELTA)
+    int ncx, ncy, ncz, dim;
+    long tnc;
+};
+
+struct DevList {
+    double re = 0.0;
+    int cap = 0;             // slots per row (slot-major storage)
+    Grid g{};
+    int32_t* cnt = nullptr;        // tp
+    int32_t* ids = nullptr;        // cap * tp, ids[l*tp + i]
+    int32_t* cell = nullptr;       // tp
+    int32_t* cell_start = nullptr; // tnc + 1
+    int32_t* cell_fill = nullptr;  // tnc
+    int32_t* order_raw = nullptr;  // tp (atomic scatter result)
+    int32_t* order = nullptr;      // tp, ids sorted by (cell, id)
+    double* xs = nullptr;          // tp positions in cell order
+    double* ys = nullptr;
+    double* zs = nullptr;
+    long tnc_alloc = 0;
+};
+
+// Lightweight by-value view of a list for kernels.
+struct ListView {
+    const int32_t* __restrict__ cnt;
+    const int32_t* __restrict__ ids;
+    long stride;             // == tp
+};
+
+struct ProfRec { int kclass; hipEvent_t a, b; };
+
+}  // namespace vox
+
+// ---- the context -----------------------------------------------------------------
+struct vox_ctx {
+    vox_params p{};
+    int tp = 0, gp = 0, wp = 0, fp = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    double* f[VOX_F_COUNT] = {};
+    int32_t* ptype = nullptr;
+    int32_t* bcon = nullptr;
+
+    vox::DevList list[2];
+    bool two_lists = false;
+    double n0s = 0.0, n0l = 0.0;
+    bool initialised = false;
+
+    // strain-rate scratch of SPS (s11 s12 s22 s13 s23 s33)
+    double* sps_s[6] = {};
+    double* scratch_d = nullptr;   // tp doubles
+
+    // PPE (rows are unknowns r = uidx[i]; slot-major with stride tp)
+    int32_t* uidx = nullptr;       // tp: particle -> unknown or -1
+    int32_t* urow = nullptr;       // tp: unknown -> particle
+    int32_t* uflag = nullptr;      // tp scan input
+    int32_t* rowlen = nullptr;     // tp
+    int32_t* acol = nullptr;       // cap_l * tp
+    double* aval = nullptr;        // cap_l * tp
+    double* adiag = nullptr;       // tp
+    double* rhs = nullptr;         // tp
+    int a_cap = 0;
+    int nu = 0;
+    int cg_expect = 0;             // iterations of the previous solve
+    // CG vectors over unknowns
+    double* cg_x = nullptr;
+    double* cg_r = nullptr;
+    double* cg_p = nullptr;
+    double* cg_q = nullptr;
+
+    // collision exact-emulation buffers
+    int32_t* coll_cnt = nullptr;   // tp
+    int32_t* coll_off = nullptr;   // tp + 1
+    int32_t* coll_i = nullptr;     // cand capacity
+    int32_t* coll_j = nullptr;
+    int32_t* coll_stamp = nullptr; // tp
+    int32_t* coll_logidx = nullptr;// tp
+    double* coll_log = nullptr;    // 8 doubles per logged particle
+    int32_t* coll_logn = nullptr;  // 1
+    long coll_cap = 0;
+    int32_t coll_epoch = 0;
+    double coll_margin = 0.1;      // fraction of coll added to the screen radius
+
+    // scan scratch
+    int32_t* scan_blocks = nullptr;
+    long scan_blocks_cap = 0;
+
+    // reductions
+    double* partials = nullptr;    // 4 * kMaxPartials
+    double* dscal = nullptr;       // SC_COUNT
+    int32_t* dflags = nullptr;     // FLAG_COUNT
+    int32_t* hflags = nullptr;     // pinned mirror
+    double* hscal = nullptr;       // pinned mirror
+
+    vox_status st{};
+
+    // profiling
+    bool prof_on = false;
+    std::vector<vox::ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_free;
+    double prof_ms[32] = {};
+    int64_t prof_n[32] = {};
+};
+
+namespace vox {
+
+// kernel classes for event profiling
+enum KClass {
+    KC_CELL = 0, KC_SORT, KC_LIST, KC_PND, KC_SPS, KC_VFRAC, KC_VISC, KC_PRED,
+    KC_BCON, KC_ASSEMBLE, KC_CG_INIT, KC_SPMV, KC_CG_UPDATE, KC_CG_P, KC_PHAT,
+    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_COUNT
+};
+const char* kclass_name(int k);
+int kclass_from_name(const char* name);
+
+void prof_begin(vox_ctx* c, int kclass, hipEvent_t* a);
+void prof_end(vox_ctx* c, int kclass, hipEvent_t a);
+int prof_drain(vox_ctx* c);
+
+struct ProfScope {
+    vox_ctx* c; int k; hipEvent_t a = nullptr;
+    ProfScope(vox_ctx* cc, int kk) : c(cc), k(kk) { if (c->prof_on) prof_begin(c, k, &a); }
+    ~ProfScope() { if (c->prof_on) prof_end(c, k, a); }
+};
+
+inline ListView view(const vox_ctx* c, int which)
+{
+    const DevList& L = (which == VOX_LIST_L && c->two_lists) ? c->list[1] : c->list[0];
+    return ListView{L.cnt, L.ids, (long)c->tp};
+}
+inline const DevList& list_of(const vox_ctx* c, int which)
+{
+    return (which == VOX_LIST_L && c->two_lists) ? c->list[1] : c->list[0];
+}
+
+// host-side helpers implemented in the .hip files
+int scan_exclusive(vox_ctx* c, const int32_t* in, int32_t* out, long n, int32_t* total_dev);
+int fill_i32(vox_ctx* c, int32_t* p, int32_t v, long n);
+int fill_f64(vox_ctx* c, double* p, double v, long n);
+int read_flags(vox_ctx* c);   // D2H of dflags + dscal into pinned mirrors (syncs)
+
+int build_neighbors(vox_ctx* c);
+int ensure_list_storage(vox_ctx* c, int which, int cap);
+
+// stages (vox_stages.hip)
+int st_density(vox_ctx* c);
+int st_pnd(vox_ctx* c, int on_star, int loopstarted);
+int st_sps(vox_ctx* c);
+int st_vfraction(vox_ctx* c);
+int st_viscosity(vox_ctx* c);
+int st_prediction(vox_ctx* c);
+int st_bcon(vox_ctx* c);
+int st_phat(vox_ctx* c);
+int st_pressgrad(vox_ctx* c);
+int st_euler(vox_ctx* c);
+int st_collision(vox_ctx* c);
+int st_prepdata(vox_ctx* c);
+int st_clamp_pressure(vox_ctx* c);
+int st_n0(vox_ctx* c);
+
+// PPE (vox_ppe.hip)
+int ppe_assemble(vox_ctx* c);
+int ppe_solve(vox_ctx* c, vox_solve_info* info);
+int ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32_t* cols,
+               double* vals, double* rhs, int32_t* ids);
+
+}  // namespace vox
+
+// ---- device helpers ------------------------------------------------------------------
+namespace vox {
+
+// Weight function W of functions.cu:32-70 (ktype 2 and 6 on the hot path).
+__device__ __forceinline__ double wfun(double r, int ktype, double re)
+{
+    double q = r / re;
+    if (ktype == 2) return (q < 1.0) ? (1 / q) - 1 : 0.0;
+    if (ktype == 6) return (q <= 1.0) ? (1 - q) * (1 - q) * (1 - q) : 0.0;
+    if (ktype == 1) {
+        if (q < 0.5) return 2.0 - 4.0 * (q * q);
+        if (q <= 1.0) return (2 * q - 2) * (2 * q - 2);
+        return 0.0;
+    }
+    if (ktype == 5) return (q <= 1.0) ? 1.5 * log(1 / (q + 0.000001)) : 0.0;
+    return 0.0;
+}
+
+// Agent-scope load that bypasses the non-coherent vector L1.
+__device__ __forceinline__ double load_agent(const double* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t load_agent_i(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Fixed-order block sum of one double per thread (blockDim == kBlock).
+__device__ __forceinline__ double block_sum(double v, double* lds)
+{
+    // wave-level tree through LDS keeps a fixed association order
+    int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] = lds[t] + lds[t + s];
+        __syncthreads();
+    }
+    double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+// Deterministic grid-wide reduction of nv doubles per thread. Each block writes
+// its partials; the last block to finish sums them in block order and calls
+// fin(totals) from thread 0. Returns true in the finishing block.
+template <int NV, class Fin>
+__device__ __forceinline__ void grid_reduce(const double (&v)[NV], double* partials,
+                                            int32_t* counter, double* lds, Fin fin)
+{
+    double bs[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], lds);
+    __shared__ int is_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[k * kMaxPartials + blockIdx.x] = bs[k];
+        __threadfence();
+        int ticket = atomicAdd(counter, 1);
+        is_last = (ticket == (int)gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    double tot[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
+            s = s + load_agent(&partials[k * kMaxPartials + b]);
+        tot[k] = block_sum(s, lds);
+    }
+    if (threadIdx.x == 0) {
+        fin(tot);
+        *counter = 0;
+        __threadfence();
+    }
+}
+
+}  // namespace vox

@@ -1,0 +1,398 @@
+// vox_neighbor.hip — hash-sort neighbour grid for gfx950.
+//
+// Replaces NEIGHBOR (functions.cu:2788-2929) and the GPU hash search of
+// neighbour_parallel.cu:344-444. The output reproduces the reference's CPU
+// list bit-exactly:
+//   cell  = int((x-Xmin)/re)+1, int((y-Ymin)/re)+1, int((z-Zmin)/(re+DELTA))+1
+//           (IEEE divide, truncation; functions.cu:2822-2828),
+//   order = stencil x-offset outer, y middle, z inner, clipped at the grid edge
+//           (2865-2880), ascending particle id inside a cell (the serial
+//           binning of 2820-2852 is a stable sort by cell),
+//   rule  = 3D: sqrt(dx^2+dy^2+dz^2) <= re+DELTA, self included (2886);
+//           2D: i != j and sqrt(dx^2+dy^2) < re (2907-2910).
+// The distance is evaluated exactly as the reference (no FMA: the library is
+// built with -ffp-contract=off); a +-1e-10 relative band around the radius
+// decides which candidates need the correctly rounded sqrt.
+//
+// Pipeline per list: key+count (atomic histogram) -> exclusive scan (cell
+// start) -> atomic scatter -> per-cell rank sort by id (one wave per cell,
+// LDS tile) with the positions copied into cell order -> per-particle stencil
+// walk writing slot-major rows ids[l*tp + i].
+#include "vox_internal.hpp"
+
+namespace vox {
+
+// ---- utility kernels ----------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* __restrict__ p, int32_t v, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long s = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = v;
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, double v, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long s = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = v;
+}
+
+int fill_i32(vox_ctx* c, int32_t* p, int32_t v, long n)
+{
+    if (n <= 0) return VOX_OK;
+    int g = grid_for(n);
+    if (g > 4096) g = 4096;
+    k_fill_i32<<<g, kBlock, 0, c->stream>>>(p, v, n);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+int fill_f64(vox_ctx* c, double* p, double v, long n)
+{
+    if (n <= 0) return VOX_OK;
+    int g = grid_for(n);
+    if (g > 4096) g = 4096;
+    k_fill_f64<<<g, kBlock, 0, c->stream>>>(p, v, n);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- exclusive scan (int32), three phases ----------------------------------------------
+constexpr int kScanItems = 4;                       // items per thread
+constexpr int kScanTile = kBlock * kScanItems;      // 1024 per block
+
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* lds, int32_t* total)
+{
+    int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+        int32_t add = (t >= off) ? lds[t - off] : 0;
+        __syncthreads();
+        lds[t] += add;
+        __syncthreads();
+    }
+    int32_t incl = lds[t];
+    *total = lds[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void k_scan_tiles(const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                             int32_t* __restrict__ block_sums, long n)
+{
+    __shared__ int32_t lds[kBlock];
+    long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int32_t tot;
+    int32_t ex = block_excl_scan(s, lds, &tot);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// single block: scan of the block sums (any count, looped)
+__global__ void k_scan_sums(int32_t* __restrict__ sums, long nb, int32_t* __restrict__ total)
+{
+    __shared__ int32_t lds[kBlock];
+    int32_t carry = 0;
+    for (long base = 0; base < nb; base += kBlock) {
+        long i = base + threadIdx.x;
+        int32_t v = (i < nb) ? sums[i] : 0;
+        int32_t tot;
+        int32_t ex = block_excl_scan(v, lds, &tot);
+        if (i < nb) sums[i] = ex + carry;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void k_scan_add(int32_t* __restrict__ out, const int32_t* __restrict__ sums, long n)
+{
+    long base = (long)blockIdx.x * kScanTile;
+    int32_t add = sums[blockIdx.x];
+    for (int k = threadIdx.x; k < kScanTile; k += kBlock) {
+        long i = base + k;
+        if (i < n) out[i] += add;
+    }
+}
+
+// out[0..n) = exclusive scan of in[0..n); out[n] = total if out has room (caller
+// passes total_dev to receive it; may alias out + n).
+int scan_exclusive(vox_ctx* c, const int32_t* in, int32_t* out, long n, int32_t* total_dev)
+{
+    long nb = (n + kScanTile - 1) / kScanTile;
+    if (nb < 1) nb = 1;
+    if (nb > c->scan_blocks_cap) {
+        if (c->scan_blocks) (void)hipFree(c->scan_blocks);
+        c->scan_blocks = nullptr;
+        VOX_HIP(hipMalloc(&c->scan_blocks, sizeof(int32_t) * (nb + 64)));
+        c->scan_blocks_cap = nb + 64;
+    }
+    k_scan_tiles<<<(int)nb, kBlock, 0, c->stream>>>(in, out, c->scan_blocks, n);
+    k_scan_sums<<<1, kBlock, 0, c->stream>>>(c->scan_blocks, nb, total_dev);
+    if (nb > 1) k_scan_add<<<(int)nb, kBlock, 0, c->stream>>>(out, c->scan_blocks, n);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- grid ------------------------------------------------------------------------------
+static Grid make_grid(const vox_params& p, double re)
+{
+    Grid g;
+    g.xmin = p.xmin; g.ymin = p.ymin; g.zmin = p.zmin;
+    g.hx = re;
+    g.hz = re + p.delta;
+    g.dim = p.dim;
+    g.ncx = (int)((p.xmax - p.xmin) / re) + 1;                    // functions.cu:2790-2791
+    g.ncy = (int)((p.ymax - p.ymin) / re) + 1;
+    g.ncz = (p.dim == 3) ? (int)((p.zmax - p.zmin) / g.hz) + 1 : 1; // 2796
+    g.tnc = (long)g.ncx * g.ncy * g.ncz;
+    return g;
+}
+
+// 1-based cell coordinates; false outside the grid (the reference has no check).
+__device__ __forceinline__ bool cell_coords(const Grid& g, double x, double y, double z,
+                                            int& ic, int& jc, int& kc)
+{
+    if (!isfinite(x) || !isfinite(y) || (g.dim == 3 && !isfinite(z))) return false;
+    double ax = (x - g.xmin) / g.hx;
+    double ay = (y - g.ymin) / g.hx;
+    if (ax > 2e9 || ay > 2e9 || ax < -2e9 || ay < -2e9) return false;
+    ic = (int)ax + 1;
+    jc = (int)ay + 1;
+    kc = 1;
+    if (g.dim == 3) {
+        double az = (z - g.zmin) / g.hz;
+        if (az > 2e9 || az < -2e9) return false;
+        kc = (int)az + 1;
+    }
+    return ic >= 1 && ic <= g.ncx && jc >= 1 && jc <= g.ncy && kc >= 1 && kc <= g.ncz;
+}
+
+__global__ void k_cell_keys(Grid g, int n, const double* __restrict__ x,
+                            const double* __restrict__ y, const double* __restrict__ z,
+                            int32_t* __restrict__ cell, int32_t* __restrict__ count,
+                            int32_t* __restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ic, jc, kc;
+    double zi = (g.dim == 3) ? z[i] : 0.0;
+    int32_t key = 0;
+    if (cell_coords(g, x[i], y[i], zi, ic, jc, kc)) {
+        key = (ic - 1) + (jc - 1) * g.ncx + (kc - 1) * g.ncx * g.ncy;   // 2826-2834, 0-based
+    } else {
+        atomicAdd(&flags[FLAG_DOMAIN], 1);
+    }
+    cell[i] = key;
+    atomicAdd(&count[key], 1);
+}
+
+__global__ void k_scatter(int n, const int32_t* __restrict__ cell,
+                          const int32_t* __restrict__ start, int32_t* __restrict__ fill,
+                          int32_t* __restrict__ order_raw)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t k = cell[i];
+    int32_t pos = start[k] + atomicAdd(&fill[k], 1);
+    order_raw[pos] = i;
+}
+
+// One wave per cell: rank of each id among the cell's ids (ids are unique), so
+// the run becomes ascending by id; positions are copied into the same order.
+constexpr int kSortTile = 512;   // ids per wave held in LDS
+
+__global__ __launch_bounds__(kBlock) void k_cell_sort(
+    long tnc, int dim, const int32_t* __restrict__ start, const int32_t* __restrict__ order_raw,
+    int32_t* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, double* __restrict__ xs, double* __restrict__ ys,
+    double* __restrict__ zs)
+{
+    __shared__ int32_t tile[kBlock / kWave][kSortTile];
+    int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    long cellid = (long)blockIdx.x * (kBlock / kWave) + wv;
+    if (cellid >= tnc) return;
+    int s = start[cellid], e = start[cellid + 1], n = e - s;
+    if (n <= 0) return;
+    const int32_t* src = order_raw + s;
+    bool in_lds = (n <= kSortTile);
+    if (in_lds)
+        for (int k = lane; k < n; k += kWave) tile[wv][k] = src[k];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int k = lane; k < n; k += kWave) {
+        int32_t id = in_lds ? tile[wv][k] : src[k];
+        int rank = 0;
+        for (int m = 0; m < n; ++m) {
+            int32_t o = in_lds ? tile[wv][m] : src[m];
+            rank += (o < id);
+        }
+        int pos = s + rank;
+        order[pos] = id;
+        xs[pos] = x[id];
+        ys[pos] = y[id];
+        if (dim == 3) zs[pos] = z[id];
+    }
+}
+
+// Per-particle stencil walk. R is the accept radius (3D: re+DELTA, 2D: re).
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_build_list(
+    Grid g, int n, double R, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int32_t* __restrict__ cell,
+    const int32_t* __restrict__ start, const int32_t* __restrict__ order,
+    const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
+    int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
+    int flag_overflow, int flag_max)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = 0;
+    if (i < n) {
+        const int32_t key = cell[i];
+        const int ncxy = g.ncx * g.ncy;
+        const int ic = key % g.ncx + 1;
+        const int jc = (key / g.ncx) % g.ncy + 1;
+        const int kc = key / ncxy + 1;
+        const int m1 = (ic == 1) ? 0 : -1, m2 = (ic == g.ncx) ? 0 : 1;
+        const int m3 = (jc == 1) ? 0 : -1, m4 = (jc == g.ncy) ? 0 : 1;
+        int m5 = 0, m6 = 0;
+        if (g.dim == 3) { m5 = (kc == 1) ? 0 : -1; m6 = (kc == g.ncz) ? 0 : 1; }
+        const double xi = x[i], yi = y[i], zi = (g.dim == 3) ? z[i] : 0.0;
+        const double R2 = R * R;
+        const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
+        const long stride = n;
+        for (int row = m1; row <= m2; ++row)
+            for (int col = m3; col <= m4; ++col)
+                for (int dep = m5; dep <= m6; ++dep) {
+                    int cn = (ic - 1 + row) + (jc - 1 + col) * g.ncx + (kc - 1 + dep) * ncxy;
+                    int s = start[cn], e = start[cn + 1];
+                    for (int jj = s; jj < e; ++jj) {
+                        double dx = xs[jj] - xi;
+                        double dy = ys[jj] - yi;
+                        double d2;
+                        if (g.dim == 3) {
+                            double dz = zs[jj] - zi;
+                            d2 = dx * dx + dy * dy + dz * dz;
+                        } else {
+                            d2 = dx * dx + dy * dy;
+                        }
+                        bool acc;
+                        if (d2 < lo) acc = true;
+                        else if (d2 > hi) acc = false;
+                        else acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                        if (!acc) continue;
+                        int j = order[jj];
+                        if (g.dim != 3 && j == i) continue;
+                        if (k < cap) ids[(long)k * stride + i] = j;
+                        ++k;
+                    }
+                }
+        cnt[i] = k;
+        if (k > cap) atomicAdd(&flags[flag_overflow], 1);
+    }
+    int m = wave_max_i32(k);
+    if ((threadIdx.x % kWave) == 0) atomicMax(&flags[flag_max], m);
+}
+
+// ---- host side --------------------------------------------------------------------------
+int ensure_list_storage(vox_ctx* c, int which, int cap)
+{
+    DevList& L = c->list[which];
+    if (cap <= L.cap && L.ids) return VOX_OK;
+    if (L.ids) (void)hipFree(L.ids);
+    L.ids = nullptr;
+    VOX_HIP(hipMalloc(&L.ids, sizeof(int32_t) * (size_t)cap * c->tp));
+    L.cap = cap;
+    return VOX_OK;
+}
+
+static int build_one(vox_ctx* c, int which, double re)
+{
+    DevList& L = c->list[which];
+    const int n = c->tp;
+    L.re = re;
+    L.g = make_grid(c->p, re);
+    if (L.g.tnc <= 0 || L.g.tnc > (1L << 30))
+        return fail(VOX_ERR_ARG, "neighbour grid has an invalid number of cells");
+    if (L.g.tnc > L.tnc_alloc) {
+        if (L.cell_start) (void)hipFree(L.cell_start);
+        if (L.cell_fill) (void)hipFree(L.cell_fill);
+        L.cell_start = L.cell_fill = nullptr;
+        VOX_HIP(hipMalloc(&L.cell_start, sizeof(int32_t) * (L.g.tnc + 1)));
+        VOX_HIP(hipMalloc(&L.cell_fill, sizeof(int32_t) * (L.g.tnc + 1)));
+        L.tnc_alloc = L.g.tnc;
+    }
+    const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
+    const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
+    {
+        ProfScope ps(c, KC_CELL);
+        VOX_TRY(fill_i32(c, L.cell_fill, 0, L.g.tnc));
+        k_cell_keys<<<grid_for(n), kBlock, 0, c->stream>>>(L.g, n, c->f[VOX_F_X], c->f[VOX_F_Y],
+                                                             c->f[VOX_F_Z], L.cell, L.cell_fill,
+                                                             c->dflags);
+        VOX_HIP(hipGetLastError());
+    }
+    {
+        ProfScope ps(c, KC_SORT);
+        VOX_TRY(scan_exclusive(c, L.cell_fill, L.cell_start, L.g.tnc, L.cell_start + L.g.tnc));
+        VOX_TRY(fill_i32(c, L.cell_fill, 0, L.g.tnc));
+        k_scatter<<<grid_for(n), kBlock, 0, c->stream>>>(n, L.cell, L.cell_start, L.cell_fill,
+                                                           L.order_raw);
+        long nb = (L.g.tnc + (kBlock / kWave) - 1) / (kBlock / kWave);
+        k_cell_sort<<<(int)nb, kBlock, 0, c->stream>>>(L.g.tnc, c->p.dim, L.cell_start,
+                                                        L.order_raw, L.order, c->f[VOX_F_X],
+                                                        c->f[VOX_F_Y], c->f[VOX_F_Z], L.xs, L.ys,
+                                                        L.zs);
+        VOX_HIP(hipGetLastError());
+    }
+    double R = (c->p.dim == 3) ? re + c->p.delta : re;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        {
+            ProfScope ps(c, KC_LIST);
+            VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
+            VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
+            k_build_list<<<grid_for(n), kBlock, 0, c->stream>>>(
+                L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
+                L.order, L.xs, L.ys, L.zs, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+            VOX_HIP(hipGetLastError());
+        }
+        VOX_TRY(read_flags(c));
+        if (c->hflags[FLAG_DOMAIN] != 0)
+            return fail(VOX_ERR_DOMAIN, "particle outside the neighbour grid box (" +
+                                            std::to_string(c->hflags[FLAG_DOMAIN]) + " particles)");
+        if (c->hflags[fo] == 0) return VOX_OK;
+        int need = c->hflags[fm];
+        int cap = ((need + need / 4) + 7) & ~7;
+        VOX_TRY(ensure_list_storage(c, which, cap));
+    }
+    return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+int build_neighbors(vox_ctx* c)
+{
+    VOX_TRY(fill_i32(c, c->dflags + FLAG_DOMAIN, 0, 1));
+    VOX_TRY(build_one(c, 0, c->p.re_s));
+    if (c->two_lists) VOX_TRY(build_one(c, 1, c->p.re_l));
+    c->st.max_count_s = c->hflags[FLAG_MAXCNT_S];
+    c->st.max_count_l = c->two_lists ? c->hflags[FLAG_MAXCNT_L] : c->hflags[FLAG_MAXCNT_S];
+    return VOX_OK;
+}
+
+}  // namespace vox

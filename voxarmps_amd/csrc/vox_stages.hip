@@ -1,0 +1,888 @@
+// vox_stages.hip — per-particle stages of the MPS time step on gfx950.
+//
+// One thread per particle; neighbour rows are slot-major (ids[l*tp + i]) so a
+// wave's 64 index loads per slot are one coalesced 256-byte request, and each
+// row is summed sequentially in the reference's order so that, built with
+// -ffp-contract=off, every per-particle sum is bit-identical to the reference's
+// sequential CPU path.
+#include "vox_internal.hpp"
+
+namespace vox {
+
+#define VOX_ROW_J(L, l, i) ((L).ids[(long)(l) * (L).stride + (i)])
+
+__device__ __forceinline__ double pdist(int dim, const double* __restrict__ x,
+                                        const double* __restrict__ y,
+                                        const double* __restrict__ z, int i, int j,
+                                        double xi, double yi, double zi)
+{
+    double dx = x[j] - xi, dy = y[j] - yi;
+    if (dim == 3) {
+        double dz = z[j] - zi;
+        return sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    return sqrt(dx * dx + dy * dy);
+}
+
+// ---- density: main.cu:970-977 -----------------------------------------------------------
+__global__ void k_density(int n, const int32_t* __restrict__ ptype, double rho1, double rho2,
+                          double* __restrict__ rho)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rho[i] = (ptype[i] == 2) ? rho2 : rho1;
+}
+
+int st_density(vox_ctx* c)
+{
+    ProfScope ps(c, KC_MISC);
+    k_density<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->ptype, c->p.rho1, c->p.rho2,
+                                                        c->f[VOX_F_RHO]);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- PNUM / PNUMSTAR: functions.cu:94-123, 157-180 -------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, double re,
+                                                double ncorr, ListView L,
+                                                const double* __restrict__ x,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ z,
+                                                double* __restrict__ out,
+                                                const double* __restrict__ p,
+                                                double* __restrict__ pnew)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    double sum = 0.0;
+    int cnt = L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(L, l, i);
+        if (j == i) continue;
+        double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
+        sum = sum + wfun(d, ktype, re);
+    }
+    if (ktype != 2) sum = sum * ncorr;
+    out[i] = sum;
+    if (pnew) pnew[i] = p[i];
+}
+
+int st_pnd(vox_ctx* c, int on_star, int loopstarted)
+{
+    ProfScope ps(c, KC_PND);
+    const double* x = c->f[on_star ? VOX_F_XSTAR : VOX_F_X];
+    const double* y = c->f[on_star ? VOX_F_YSTAR : VOX_F_Y];
+    const double* z = c->f[on_star ? VOX_F_ZSTAR : VOX_F_Z];
+    double* out = c->f[on_star ? VOX_F_NSTAR : VOX_F_N];
+    // PNUMSTAR applies no Ncorrection: multiplying by 1.0 is exact
+    double ncorr = on_star ? 1.0 : c->p.ncorrection;
+    k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
+        c->tp, c->p.dim, c->p.ktype, c->p.re_s, ncorr, view(c, VOX_LIST_S), x, y, z, out,
+        c->f[VOX_F_P], (!on_star && !loopstarted) ? c->f[VOX_F_PNEW] : nullptr);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// n0 = max n over list S (main.cu:1011) and the same maximum of the PND on list L.
+int st_n0(vox_ctx* c)
+{
+    std::vector<double> h(c->tp);
+    VOX_HIP(hipMemcpyAsync(h.data(), c->f[VOX_F_N], sizeof(double) * c->tp,
+                           hipMemcpyDeviceToHost, c->stream));
+    VOX_HIP(hipStreamSynchronize(c->stream));
+    double mx = 0.0;
+    for (double v : h) if (v > mx) mx = v;
+    c->n0s = mx;
+    if (c->two_lists) {
+        k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
+            c->tp, c->p.dim, c->p.ktype, c->p.re_l, c->p.ncorrection, view(c, VOX_LIST_L),
+            c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], c->scratch_d, nullptr, nullptr);
+        VOX_HIP(hipGetLastError());
+        VOX_HIP(hipMemcpyAsync(h.data(), c->scratch_d, sizeof(double) * c->tp,
+                               hipMemcpyDeviceToHost, c->stream));
+        VOX_HIP(hipStreamSynchronize(c->stream));
+        double ml = 0.0;
+        for (double v : h) if (v > ml) ml = v;
+        c->n0l = ml;
+    } else {
+        c->n0l = mx;
+    }
+    return VOX_OK;
+}
+
+// ---- strain rate: SPS functions.cu:1738-1790, VISCOSITY 2095-2150 ---------------------------------
+struct Strain { double s11, s12, s22, s13, s23, s33; };
+
+__device__ __forceinline__ Strain strain_row(int i, int dim, int ktype, double re, double n0,
+                                             double dmin, bool use_dmin, ListView L,
+                                             const double* __restrict__ x,
+                                             const double* __restrict__ y,
+                                             const double* __restrict__ z,
+                                             const double* __restrict__ u,
+                                             const double* __restrict__ v,
+                                             const double* __restrict__ w)
+{
+    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    double ui = u[i], vi = v[i], wi = (dim == 3) ? w[i] : 0.0;
+    double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
+    int cnt = L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(L, l, i);
+        if (j == i) continue;
+        double dx = x[j] - xi, dy = y[j] - yi, dz = 0.0, d;
+        if (dim == 3) { dz = z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
+        else d = sqrt(dx * dx + dy * dy);
+        if (use_dmin && !(d > dmin)) continue;
+        double wv = wfun(d, ktype, re);
+        double du = u[j] - ui, dv = v[j] - vi;
+        s1 = s1 + du * dx * wv / d / d;
+        s2 = s2 + dv * dy * wv / d / d;
+        s3 = s3 + du * dy * wv / d / d;
+        s4 = s4 + dv * dx * wv / d / d;
+        if (dim == 3) {
+            double dw = w[j] - wi;
+            s5 = s5 + dw * dz * wv / d / d;
+            s6 = s6 + du * dz * wv / d / d;
+            s7 = s7 + dv * dz * wv / d / d;
+            s8 = s8 + dw * dx * wv / d / d;
+            s9 = s9 + dw * dy * wv / d / d;
+        }
+    }
+    double Uxx = (2.0 / n0) * s1, Uyy = (2.0 / n0) * s2, Uxy = (2.0 / n0) * s3, Uyx = (2.0 / n0) * s4;
+    Strain S;
+    S.s12 = 0.5 * (Uxy + Uyx);
+    S.s11 = 0.5 * (Uxx + Uxx);
+    S.s22 = 0.5 * (Uyy + Uyy);
+    S.s13 = S.s23 = S.s33 = 0.0;
+    if (dim == 3) {
+        double Uzz = (2.0 / n0) * s5, Uxz = (2.0 / n0) * s6, Uyz = (2.0 / n0) * s7;
+        double Uzx = (2.0 / n0) * s8, Uzy = (2.0 / n0) * s9;
+        S.s33 = 0.5 * (Uzz + Uzz);
+        S.s13 = 0.5 * (Uxz + Uzx);
+        S.s23 = 0.5 * (Uyz + Uzy);
+    }
+    return S;
+}
+
+// ---- SPS: functions.cu:1719-1841 -------------------------------------------------------------
+struct SpsArgs {
+    int n, first, dim, ktype;
+    double re, n0, dmin, csdl2;
+    ListView L;
+    const double *x, *y, *z, *u, *v, *w;
+    double *s11, *s12, *s22, *s13, *s23, *s33, *neut, *t1, *t2, *t3;
+};
+
+__global__ __launch_bounds__(kBlock) void k_sps1(SpsArgs a)
+{
+    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    Strain S = strain_row(i, a.dim, a.ktype, a.re, a.n0, a.dmin, true, a.L, a.x, a.y, a.z, a.u,
+                          a.v, a.w);
+    a.s11[i] = S.s11; a.s12[i] = S.s12; a.s22[i] = S.s22;
+    a.s13[i] = S.s13; a.s23[i] = S.s23; a.s33[i] = S.s33;
+    double s = sqrt(2 * S.s11 * S.s11) + 2 * sqrt(2 * S.s12 * S.s12) + sqrt(2 * S.s22 * S.s22);
+    if (a.dim == 3)
+        s = s + sqrt(2 * S.s33 * S.s33) + 2 * sqrt(2 * S.s13 * S.s13) + 2 * sqrt(2 * S.s23 * S.s23);
+    a.neut[i] = a.csdl2 * s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sps2(SpsArgs a)
+{
+    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double xi = a.x[i], yi = a.y[i], zi = (a.dim == 3) ? a.z[i] : 0.0, ni = a.neut[i];
+    double s1 = 0, s2 = 0, s3 = 0;
+    int cnt = a.L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(a.L, l, i);
+        if (j == i) continue;
+        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, d;
+        if (a.dim == 3) { dz = a.z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
+        else d = sqrt(dx * dx + dy * dy);
+        if (!(d > a.dmin)) continue;
+        double wv = wfun(d, a.ktype, a.re);
+        double dn = a.neut[j] - ni;
+        s1 = s1 + dn * dx * wv / d / d;
+        s2 = s2 + dn * dy * wv / d / d;
+        if (a.dim == 3) s3 = s3 + dn * dz * wv / d / d;
+    }
+    double s11 = a.s11[i], s12 = a.s12[i], s22 = a.s22[i];
+    if (a.dim == 3) {
+        double s13 = a.s13[i], s23 = a.s23[i], s33 = a.s33[i];
+        a.t1[i] = (2.0 / a.n0) * (2 * s11 * s1 + 2 * s12 * s2 + 2 * s13 * s3);
+        a.t2[i] = (2.0 / a.n0) * (2 * s12 * s1 + 2 * s22 * s2 + 2 * s23 * s3);
+        a.t3[i] = (2.0 / a.n0) * (2 * s13 * s1 + 2 * s23 * s2 + 2 * s33 * s3);
+    } else {
+        a.t1[i] = (2.0 / a.n0) * (2 * s11 * s1 + 2 * s12 * s2);
+        a.t2[i] = (2.0 / a.n0) * (2 * s12 * s1 + 2 * s22 * s2);
+    }
+}
+
+int st_sps(vox_ctx* c)
+{
+    ProfScope ps(c, KC_SPS);
+    SpsArgs a;
+    a.n = c->tp; a.first = c->gp; a.dim = c->p.dim; a.ktype = c->p.ktype;
+    a.re = c->p.re_s; a.n0 = c->n0s; a.dmin = c->p.coll / 10;
+    a.csdl2 = (c->p.cs * c->p.dl) * (c->p.cs * c->p.dl);
+    a.L = view(c, VOX_LIST_S);
+    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+    a.u = c->f[VOX_F_UNEW]; a.v = c->f[VOX_F_VNEW]; a.w = c->f[VOX_F_WNEW];
+    a.s11 = c->sps_s[0]; a.s12 = c->sps_s[1]; a.s22 = c->sps_s[2];
+    a.s13 = c->sps_s[3]; a.s23 = c->sps_s[4]; a.s33 = c->sps_s[5];
+    a.neut = c->f[VOX_F_NEUT];
+    a.t1 = c->f[VOX_F_TURB1]; a.t2 = c->f[VOX_F_TURB2]; a.t3 = c->f[VOX_F_TURB3];
+    int m = c->tp - c->gp;
+    if (m <= 0) return VOX_OK;
+    k_sps1<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    k_sps2<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- V_FRACTION, method 2: functions.cu:2604-2634 ---------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_vfraction(int n, int dim, int ktype, double re,
+                                                      ListView L, const int32_t* __restrict__ pt,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      double* __restrict__ C)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    double s1 = 0, s2 = 0;
+    int cnt = L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(L, l, i);
+        if (j == i) continue;
+        int tj = pt[j];
+        if (tj <= 0) continue;
+        double wv = wfun(pdist(dim, x, y, z, i, j, xi, yi, zi), ktype, re);
+        s1 = s1 + wv;
+        if (tj == 2) s2 = s2 + wv;
+    }
+    double cv = s2 / s1;
+    if (s1 == 0) cv = 0;
+    C[i] = cv;
+}
+
+int st_vfraction(vox_ctx* c)
+{
+    ProfScope ps(c, KC_VFRAC);
+    k_vfraction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
+        c->tp, c->p.dim, c->p.ktype, c->p.re_s, view(c, VOX_LIST_S), c->ptype, c->f[VOX_F_X],
+        c->f[VOX_F_Y], c->f[VOX_F_Z], c->f[VOX_F_C]);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- VISCOSITY: functions.cu:2053-2184 ----------------------------------------------------------
+__global__ void k_visc_newton(int n, const int32_t* __restrict__ pt, double m1, double m2,
+                              double* __restrict__ meu)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int t = pt[i];
+    if (t <= 1) meu[i] = m1;
+    if (t == 2) meu[i] = m2;
+}
+
+struct HbArgs {
+    int n, dim, ktype;
+    double re, n0, m1, cohes, phi_param, meu0, npl;
+    ListView L;
+    const int32_t* pt;
+    const double *x, *y, *z, *u, *v, *w, *phat;
+    double* meu;
+};
+
+__global__ __launch_bounds__(kBlock) void k_visc_hb(HbArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    if (a.pt[i] == 1) { a.meu[i] = a.m1; return; }
+    Strain S = strain_row(i, a.dim, a.ktype, a.re, a.n0, 0.0, false, a.L, a.x, a.y, a.z, a.u, a.v,
+                          a.w);
+    // phi reads C[I] with the caller's I == 0 (main.cu:211): always phi = 0 (2154-2155)
+    double cz = 0.0;
+    double phi = (cz - 0.25) * a.phi_param / (1 - 0.25);
+    if (cz < 0.25) phi = 0;
+    double ys = a.cohes * cos(phi) + a.phat[i] * sin(phi);
+    double II;
+    if (a.dim == 3)
+        II = fabs((S.s11 * S.s22 * S.s33 + S.s12 * S.s23 * S.s13 + S.s13 * S.s12 * S.s23) -
+                  (S.s13 * S.s22 * S.s13 + S.s11 * S.s23 * S.s23 + S.s12 * S.s12 * S.s33));
+    else
+        II = fabs(S.s11 * S.s22 - S.s12 * S.s12);
+    double m = ys / 2 / sqrt(II) + a.meu0 * pow(4 * II, (a.npl - 1) / 2);
+    if (II == 0 || m > 200) m = 200;
+    a.meu[i] = m;
+}
+
+int st_viscosity(vox_ctx* c)
+{
+    ProfScope ps(c, KC_VISC);
+    const vox_params& p = c->p;
+    if (p.fluid2_type == 0) {
+        k_visc_newton<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->ptype, p.nu1 * p.rho1,
+                                                                 p.nu2 * p.rho2, c->f[VOX_F_MEU]);
+    } else {
+        HbArgs a;
+        a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype; a.re = p.re_s; a.n0 = c->n0s;
+        a.m1 = p.nu1 * p.rho1; a.cohes = p.cohes; a.phi_param = p.phi; a.meu0 = p.meu0;
+        a.npl = p.n_pl; a.L = view(c, VOX_LIST_S); a.pt = c->ptype;
+        a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+        a.u = c->f[VOX_F_UNEW]; a.v = c->f[VOX_F_VNEW]; a.w = c->f[VOX_F_WNEW];
+        a.phat = c->f[VOX_F_PHAT]; a.meu = c->f[VOX_F_MEU];
+        k_visc_hb<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    }
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- PREDICTION: functions.cu:2283-2369 ------------------------------------------------------------
+struct PredArgs {
+    int n, dim, ktype;
+    double re_s, re_l, rho1, rho2, gx, gy, gz, dt, lapc, ln0, gcoef_num, n0s, relax1;
+    ListView L;
+    const int32_t* pt;
+    const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
+    double *xs, *ys, *zs, *us, *vs, *ws;
+};
+
+__global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int dim = a.dim;
+    const int ti = a.pt[i];
+    if (ti <= 0) {
+        a.xs[i] = a.x[i]; a.ys[i] = a.y[i]; a.us[i] = a.u[i]; a.vs[i] = a.v[i];
+        if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
+        return;
+    }
+    double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
+    double ui = a.u[i], vi = a.v[i], wi = (dim == 3) ? a.w[i] : 0.0;
+    double mi = a.meu[i], phi_ = a.ph[i];
+    double rhoN = (ti == 1) ? a.rho1 : a.rho2;
+    double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
+    int cnt = a.L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(a.L, l, i);
+        if (j == i) continue;
+        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
+        if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
+        else D = sqrt(dx * dx + dy * dy);
+        if (!(D > 0)) continue;
+        double NEU;
+        if (ti == a.pt[j]) NEU = mi / rhoN;
+        else {
+            double mj = a.meu[j];
+            NEU = 2 * mi * mj / (mi + mj) / rhoN;
+        }
+        double wg = wfun(D, a.ktype, a.re_s);
+        double wl = (a.re_l == a.re_s) ? wg : wfun(D, a.ktype, a.re_l);
+        double dp = a.pn[j] - phi_;
+        s1 = s1 + dp * dx * wg / D / D;
+        s2 = s2 + dp * dy * wg / D / D;
+        if (dim == 3) s3 = s3 + dp * dz * wg / D / D;
+        double du = a.u[j] - ui, dv = a.v[j] - vi;
+        s4 = s4 + wl * du * NEU;
+        s5 = s5 + wl * dv * NEU;
+        double dw = 0.0;
+        if (dim == 3) { dw = a.w[j] - wi; s6 = s6 + wl * dw * NEU; }
+        s7 = s7 + wl * du;
+        s8 = s8 + wl * dv;
+        if (dim == 3) s9 = s9 + wl * dw;
+    }
+    double nt = a.neut[i];
+    double gc = a.relax1 * (a.gcoef_num * a.dt / a.n0s / a.rho[i]);
+    double dt = a.dt;
+    double us = ui + a.gx * dt + a.lapc * dt * (s4 + nt * s7) / a.ln0 + dt * a.t1[i] - gc * s1;
+    double vs = vi + a.gy * dt + a.lapc * dt * (s5 + nt * s8) / a.ln0 + dt * a.t2[i] - gc * s2;
+    a.us[i] = us;
+    a.vs[i] = vs;
+    a.xs[i] = xi + dt * us;
+    a.ys[i] = yi + dt * vs;
+    if (dim == 3) {
+        double ws = wi + a.gz * dt + a.lapc * dt * (s6 + nt * s9) / a.ln0 + dt * a.t3[i] - gc * s3;
+        a.ws[i] = ws;
+        a.zs[i] = zi + dt * ws;
+    }
+}
+
+int st_prediction(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PRED);
+    const vox_params& p = c->p;
+    PredArgs a;
+    a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype; a.re_s = p.re_s; a.re_l = p.re_l;
+    a.rho1 = p.rho1; a.rho2 = p.rho2; a.gx = p.gx; a.gy = p.gy; a.gz = p.gz; a.dt = p.dt;
+    a.lapc = p.coef2d ? 4.0 : 2.0 * p.dim;
+    a.ln0 = p.lambda * c->n0l;
+    a.gcoef_num = p.coef2d ? 2.0 : (double)p.dim;
+    a.n0s = c->n0s;
+    a.relax1 = (1 - p.relaxp);
+    a.L = view(c, VOX_LIST_L); a.pt = c->ptype;
+    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+    a.u = c->f[VOX_F_U]; a.v = c->f[VOX_F_V]; a.w = c->f[VOX_F_W];
+    a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.meu = c->f[VOX_F_MEU];
+    a.neut = c->f[VOX_F_NEUT]; a.rho = c->f[VOX_F_RHO];
+    a.t1 = c->f[VOX_F_TURB1]; a.t2 = c->f[VOX_F_TURB2]; a.t3 = c->f[VOX_F_TURB3];
+    a.xs = c->f[VOX_F_XSTAR]; a.ys = c->f[VOX_F_YSTAR]; a.zs = c->f[VOX_F_ZSTAR];
+    a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
+    k_prediction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- BCON: functions.cu:433-445 -----------------------------------------------------------------
+__global__ void k_bcon(int n, const int32_t* __restrict__ pt, const double* __restrict__ ns,
+                       double n0, double dirichlet, int32_t* __restrict__ bcon)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int b;
+    if (pt[i] <= -1) b = -1;
+    else if (ns[i] / n0 < dirichlet) b = 1;
+    else b = 0;
+    bcon[i] = b;
+}
+
+int st_bcon(vox_ctx* c)
+{
+    ProfScope ps(c, KC_BCON);
+    k_bcon<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->ptype, c->f[VOX_F_NSTAR], c->n0s,
+                                                     c->p.dirichlet, c->bcon);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- pressure clamp of PRESSURECALC: functions.cu:2534-2547 --------------------------------------
+__global__ void k_clamp(int n, double pmin, double pmax, double* __restrict__ pn)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double v = pn[i];
+    if (v < pmin) v = pmin;
+    if (v > pmax || v * 0.0 != 0.0) v = pmax;
+    pn[i] = v;
+}
+
+int st_clamp_pressure(vox_ctx* c)
+{
+    ProfScope ps(c, KC_MISC);
+    k_clamp<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.pmin, c->p.pmax,
+                                                      c->f[VOX_F_PNEW]);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- PHATCALC: functions.cu:207-224 -------------------------------------------------------------
+__global__ void k_phat(int n, ListView L, const double* __restrict__ pn, double* __restrict__ ph)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double mn = 999999999999999.0;
+    int cnt = L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        double v = pn[VOX_ROW_J(L, l, i)];
+        if (v < mn) mn = v;
+    }
+    ph[i] = mn;
+}
+
+int st_phat(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PHAT);
+    k_phat<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, view(c, VOX_LIST_S),
+                                                     c->f[VOX_F_PNEW], c->f[VOX_F_PHAT]);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- PRESSGRAD: functions.cu:245-355 --------------------------------------------------------------
+struct GradArgs {
+    int first, n, dim, ktype, kh;
+    double re, gnum, dt, n0, relaxp, vmax;
+    ListView L;
+    const double *x, *y, *z, *pn, *ph, *rho, *us, *vs, *ws;
+    double *un, *vn, *wn;
+};
+
+__global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
+{
+    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int dim = a.dim;
+    double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
+    double pi_ = a.pn[i], phi_ = a.ph[i];
+    double s1 = 0, s2 = 0, s3 = 0;
+    int cnt = a.L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        int j = VOX_ROW_J(a.L, l, i);
+        if (j == i) continue;
+        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
+        if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
+        else D = sqrt(dx * dx + dy * dy);
+        double wv = wfun(D, a.ktype, a.re);
+        double dp = (a.kh == 1) ? (a.pn[j] + pi_ - (phi_ + a.ph[j])) : (a.pn[j] - phi_);
+        s1 = s1 + dp * dx * wv / D / D;
+        s2 = s2 + dp * dy * wv / D / D;
+        if (dim == 3) s3 = s3 + dp * dz * wv / D / D;
+    }
+    double g = a.relaxp * (a.gnum * a.dt / a.n0 / a.rho[i]);
+    double un = a.us[i] - g * s1;
+    double vn = a.vs[i] - g * s2;
+    double cf = 2.0;
+    if (fabs(un) > cf * a.vmax) un = a.vmax;                   // damper 345-351
+    if (vn > cf * a.vmax) vn = cf * a.vmax;
+    if (vn < -cf * a.vmax) vn = -cf * a.vmax;
+    a.un[i] = un;
+    a.vn[i] = vn;
+    if (dim == 3) {
+        double wn = a.ws[i] - g * s3;
+        if (fabs(wn) > cf * a.vmax) wn = a.vmax;
+        a.wn[i] = wn;
+    }
+}
+
+int st_pressgrad(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PGRAD);
+    const vox_params& p = c->p;
+    GradArgs a;
+    a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype;
+    a.kh = p.kh_correction; a.re = p.re_s; a.gnum = p.coef2d ? 2.0 : (double)p.dim;
+    a.dt = p.dt; a.n0 = c->n0s; a.relaxp = p.relaxp; a.vmax = p.vmax;
+    a.L = view(c, VOX_LIST_S);
+    a.x = c->f[VOX_F_XSTAR]; a.y = c->f[VOX_F_YSTAR]; a.z = c->f[VOX_F_ZSTAR];
+    a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.rho = c->f[VOX_F_RHO];
+    a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
+    a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW];
+    int m = c->tp - a.first;
+    if (m <= 0) return VOX_OK;
+    k_pressgrad<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- EULERINTEGRATION + BOUNDARIE: functions.cu:1963-2021 -------------------------------------------
+struct EulerArgs {
+    int first, n, dim, kind;
+    double dt, dl, cc, xmin, xmax, ymin, ymax, zmin, zmax;
+    double *x, *y, *z, *u, *v, *w;
+};
+
+__global__ void k_euler(EulerArgs a)
+{
+    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    double x = a.x[i] + a.u[i] * a.dt;
+    double y = a.y[i] + a.v[i] * a.dt;
+    double u = a.u[i], v = a.v[i];
+    double z = 0.0, w = 0.0;
+    if (a.dim == 3) { w = a.w[i]; z = a.z[i] + w * a.dt; }
+    const double DL = a.dl, CC = a.cc;
+    if (a.kind == VOX_TEST_DAM) {
+        if (y < a.ymin + DL * 3.0) { v = fabs(v) * CC; y = a.ymin + DL * 3.0; }
+        if (y > a.ymax - DL * 3.0) { v = -fabs(v) * CC; y = a.ymax - DL * 3.0; }
+        if (x < a.xmin + DL * 3.0) { u = fabs(u) * CC; x = a.xmin + DL * 3.0; }
+        if (x > a.xmax - DL * 3.0) { u = -fabs(u) * CC; x = a.xmax - DL * 3.0; }
+        if (a.dim == 3) {
+            if (z < a.zmin + DL * 3.0) { w = fabs(w) * CC; z = a.zmin + DL * 3.0; }
+            if (z > a.zmax - DL * 3.0) { w = -fabs(w) * CC; z = a.zmax - DL * 3.0; }
+        }
+    } else if (a.kind == VOX_TEST_TURB) {
+        if (y < a.ymin + DL * 2.0) { v = fabs(v) * CC; y = a.ymin + DL * 2.0; }
+        if (y > a.ymax - DL * 2.0) { v = -fabs(v) * CC; y = a.ymax - DL * 2.0; }
+        if (x > a.xmax + DL * 0.0) { u = (u) * CC; x = a.xmin; }
+    }
+    a.x[i] = x; a.y[i] = y; a.u[i] = u; a.v[i] = v;
+    if (a.dim == 3) { a.z[i] = z; a.w[i] = w; }
+}
+
+int st_euler(vox_ctx* c)
+{
+    ProfScope ps(c, KC_EULER);
+    const vox_params& p = c->p;
+    EulerArgs a;
+    a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim; a.kind = p.test_kind;
+    a.dt = p.dt; a.dl = p.dl; a.cc = p.vel_cor;
+    a.xmin = p.xmin; a.xmax = p.xmax; a.ymin = p.ymin; a.ymax = p.ymax; a.zmin = p.zmin; a.zmax = p.zmax;
+    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+    a.u = c->f[VOX_F_UNEW]; a.v = c->f[VOX_F_VNEW]; a.w = c->f[VOX_F_WNEW];
+    int m = c->tp - a.first;
+    if (m <= 0) return VOX_OK;
+    k_euler<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- COLLISION2: functions.cu:1523-1601 ------------------------------------------------------------
+// The reference updates particles i and j in place while it walks i = 1..TP and
+// each row in order, so a later pair sees earlier updates. That order is
+// reproduced exactly: a parallel screen lists every (i, j) with d < coll+margin
+// in (i, slot) order; one lane then replays exactly the reference loop over
+// those candidates with the live positions. A pair outside the screen can only
+// reach d < coll if its particles move by more than margin in total during the
+// pass; the replay accumulates each touched particle's path length and raises
+// a flag if any exceeds margin/2 — then an undo log restores the state and the
+// pass repeats with a wider screen.
+__global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double rc, ListView L,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        const int32_t* __restrict__ off,
+                                                        int32_t* __restrict__ cnt,
+                                                        int32_t* __restrict__ ci,
+                                                        int32_t* __restrict__ cj, long cap)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    long pos = off ? off[i] : 0;
+    int k = 0;
+    int rows = L.cnt[i];
+    for (int l = 0; l < rows; ++l) {
+        int j = VOX_ROW_J(L, l, i);
+        if (j == i) continue;
+        double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
+        if (!(d < rc)) continue;
+        if (off && pos + k < cap) { ci[pos + k] = i; cj[pos + k] = j; }
+        ++k;
+    }
+    if (!off) cnt[i] = k;
+}
+
+struct CollArgs {
+    int dim;
+    double coll, cc, rho1, rho2, dt, half_margin;
+    const int32_t* total;
+    const int32_t *ci, *cj, *pt;
+    long cap;
+    double *x, *y, *z, *u, *v, *w;
+    int32_t *stamp, *logidx, *logn, *flags;
+    double* log;
+    int32_t epoch;
+};
+
+__global__ void k_coll_replay(CollArgs a)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int total = *a.total;
+    int nlog = 0, hits = 0;
+    const int dim = a.dim;
+    for (int k = 0; k < total && k < a.cap; ++k) {
+        const int i = a.ci[k], j = a.cj[k];
+        const int ti = a.pt[i], tj = a.pt[j];
+        const double m1 = (ti == 1) ? a.rho1 : a.rho2;
+        const double m2 = (tj == 1) ? a.rho1 : a.rho2;
+        double dx = a.x[j] - a.x[i], dy = a.y[j] - a.y[i], dz = 0.0, d;
+        if (dim == 3) { dz = a.z[j] - a.z[i]; d = sqrt(dx * dx + dy * dy + dz * dz); }
+        else d = sqrt(dx * dx + dy * dy);
+        if (!(d < a.coll)) continue;
+        double ug = (m1 * a.u[i] + m2 * a.u[j]) / (m1 + m2);
+        double vg = (m1 * a.v[i] + m2 * a.v[j]) / (m1 + m2);
+        double wg = 0, wr = 0, wm = 0;
+        if (dim == 3) wg = (m1 * a.w[i] + m2 * a.w[j]) / (m1 + m2);
+        double ur = m1 * (a.u[i] - ug);
+        double vr = m1 * (a.v[i] - vg);
+        if (dim == 3) wr = m1 * (a.w[i] - wg);
+        double vabs = (dim == 3) ? (ur * dx + vr * dy + wr * dz) / d : (ur * dx + vr * dy) / d;
+        double um = (1.0 + a.cc) * vabs * dx / d;
+        double vm = (1.0 + a.cc) * vabs * dy / d;
+        if (dim == 3) wm = (1.0 + a.cc) * vabs * dz / d;
+        if (!(vabs > 0)) continue;
+        ++hits;
+        for (int side = 0; side < 2; ++side) {
+            const int q = side ? j : i;
+            if ((side ? tj : ti) <= 0) continue;
+            if (a.stamp[q] != a.epoch) {
+                a.stamp[q] = a.epoch;
+                a.logidx[q] = nlog;
+                double* e = a.log + 8L * nlog;
+                e[0] = (double)q; e[1] = a.x[q]; e[2] = a.y[q]; e[3] = a.u[q]; e[4] = a.v[q];
+                e[5] = (dim == 3) ? a.z[q] : 0.0; e[6] = (dim == 3) ? a.w[q] : 0.0; e[7] = 0.0;
+                ++nlog;
+            }
+            const double m = side ? m2 : m1;
+            const double sg = side ? 1.0 : -1.0;
+            double nx, ny, nz = 0.0;
+            if (!side) {
+                a.u[q] = a.u[q] - um / m;
+                a.v[q] = a.v[q] - vm / m;
+                nx = a.x[q] - a.dt * um / m;
+                ny = a.y[q] - a.dt * vm / m;
+            } else {
+                a.u[q] = a.u[q] + um / m;
+                a.v[q] = a.v[q] + vm / m;
+                nx = a.x[q] + a.dt * um / m;
+                ny = a.y[q] + a.dt * vm / m;
+            }
+            double step = fabs(nx - a.x[q]) + fabs(ny - a.y[q]);
+            a.x[q] = nx; a.y[q] = ny;
+            if (dim == 3) {
+                if (!side) { a.w[q] = a.w[q] - wm / m; nz = a.z[q] - a.dt * wm / m; }
+                else { a.w[q] = a.w[q] + wm / m; nz = a.z[q] + a.dt * wm / m; }
+                step += fabs(nz - a.z[q]);
+                a.z[q] = nz;
+            }
+            (void)sg;
+            a.log[8L * a.logidx[q] + 7] += step;
+        }
+    }
+    int viol = 0;
+    for (int k = 0; k < nlog; ++k)
+        if (a.log[8L * k + 7] > a.half_margin) viol = 1;
+    a.flags[FLAG_COLL_HITS] = hits;
+    a.flags[FLAG_COLL_CAND] = total;
+    a.flags[FLAG_COLL_VIOLATION] = viol;
+    *a.logn = nlog;
+}
+
+__global__ void k_coll_undo(int dim, const int32_t* __restrict__ logn, const double* __restrict__ log,
+                            double* x, double* y, double* z, double* u, double* v, double* w)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *logn) return;
+    const double* e = log + 8L * k;
+    int q = (int)e[0];
+    x[q] = e[1]; y[q] = e[2]; u[q] = e[3]; v[q] = e[4];
+    if (dim == 3) { z[q] = e[5]; w[q] = e[6]; }
+}
+
+static int coll_reserve(vox_ctx* c, long need)
+{
+    if (need <= c->coll_cap) return VOX_OK;
+    long cap = need + need / 2 + 1024;
+    if (c->coll_i) (void)hipFree(c->coll_i);
+    if (c->coll_j) (void)hipFree(c->coll_j);
+    if (c->coll_log) (void)hipFree(c->coll_log);
+    c->coll_i = c->coll_j = nullptr;
+    c->coll_log = nullptr;
+    VOX_HIP(hipMalloc(&c->coll_i, sizeof(int32_t) * cap));
+    VOX_HIP(hipMalloc(&c->coll_j, sizeof(int32_t) * cap));
+    VOX_HIP(hipMalloc(&c->coll_log, sizeof(double) * 8 * (2 * cap)));
+    c->coll_cap = cap;
+    return VOX_OK;
+}
+
+int st_collision(vox_ctx* c)
+{
+    ProfScope ps(c, KC_COLL);
+    const vox_params& p = c->p;
+    const int n = c->tp;
+    double* X = c->f[VOX_F_X]; double* Y = c->f[VOX_F_Y]; double* Z = c->f[VOX_F_Z];
+    double* U = c->f[VOX_F_UNEW]; double* V = c->f[VOX_F_VNEW]; double* W = c->f[VOX_F_WNEW];
+    ListView L = view(c, VOX_LIST_S);
+    for (int attempt = 0; attempt < 6; ++attempt) {
+        double margin = p.coll * c->coll_margin;
+        double rc = p.coll + margin;
+        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, nullptr,
+                                                              c->coll_cnt, nullptr, nullptr, 0);
+        VOX_TRY(scan_exclusive(c, c->coll_cnt, c->coll_off, n, c->coll_off + n));
+        int32_t total = 0;
+        VOX_HIP(hipMemcpyAsync(&total, c->coll_off + n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               c->stream));
+        VOX_HIP(hipStreamSynchronize(c->stream));
+        c->st.collision_cand = total;
+        if (total == 0) { c->st.collisions = 0; return VOX_OK; }
+        VOX_TRY(coll_reserve(c, total));
+        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, c->coll_off,
+                                                              nullptr, c->coll_i, c->coll_j,
+                                                              c->coll_cap);
+        if (++c->coll_epoch == INT32_MAX) {
+            VOX_TRY(fill_i32(c, c->coll_stamp, 0, n));
+            c->coll_epoch = 1;
+        }
+        CollArgs a;
+        a.dim = p.dim; a.coll = p.coll; a.cc = p.cc; a.rho1 = p.rho1; a.rho2 = p.rho2; a.dt = p.dt;
+        // Once the screen radius covers the list's own accept radius every
+        // listed pair is a candidate and the replay is exact for any motion.
+        const double list_r = (p.dim == 3) ? p.re_s + p.delta : p.re_s;
+        a.half_margin = (rc > list_r) ? HUGE_VAL : 0.45 * margin;
+        a.total = c->coll_off + n; a.ci = c->coll_i; a.cj = c->coll_j; a.pt = c->ptype;
+        a.cap = c->coll_cap;
+        a.x = X; a.y = Y; a.z = Z; a.u = U; a.v = V; a.w = W;
+        a.stamp = c->coll_stamp; a.logidx = c->coll_logidx; a.logn = c->coll_logn;
+        a.flags = c->dflags; a.log = c->coll_log; a.epoch = c->coll_epoch;
+        k_coll_replay<<<1, kWave, 0, c->stream>>>(a);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(read_flags(c));
+        c->st.collisions = c->hflags[FLAG_COLL_HITS];
+        if (!c->hflags[FLAG_COLL_VIOLATION]) return VOX_OK;
+        // undo and widen the screen
+        k_coll_undo<<<grid_for(2 * total), kBlock, 0, c->stream>>>(p.dim, c->coll_logn, c->coll_log,
+                                                                  X, Y, Z, U, V, W);
+        VOX_HIP(hipGetLastError());
+        c->coll_margin *= 4.0;
+    }
+    return fail(VOX_ERR_SOLVER, "collision screen could not bound particle motion");
+}
+
+// ---- PREPDATA: functions.cu:2688-2746 --------------------------------------------------------------
+struct PrepArgs {
+    int n, first, dim;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    const double *x, *y, *z, *un, *vn, *wn, *pn;
+    double *u, *v, *w, *p;
+    int32_t* flags;
+};
+
+__global__ __launch_bounds__(kBlock) void k_prepdata(PrepArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int nanc = 0, oob = 0;
+    if (i < a.n) {
+        double pv = a.pn[i];
+        a.p[i] = pv;
+        double u, v, w = 0.0;
+        if (i >= a.first) {
+            u = a.un[i]; v = a.vn[i];
+            a.u[i] = u; a.v[i] = v;
+            if (a.dim == 3) { w = a.wn[i]; a.w[i] = w; }
+        } else {
+            u = a.u[i]; v = a.v[i];
+            if (a.dim == 3) w = a.w[i];
+        }
+        double x = a.x[i], y = a.y[i];
+        if (a.dim == 3) {
+            double z = a.z[i];
+            if (pv * 0 != 0 || u * 0 != 0 || v * 0 != 0 || w * 0 != 0) nanc = 1;
+            if (x >= a.xmax || y >= a.ymax || z >= a.zmax || x <= a.xmin || y <= a.ymin || z <= a.zmin)
+                oob = 1;
+        } else {
+            if (pv * 0 != 0 || u * 0 != 0 || v * 0 != 0) nanc = 1;
+            if (x >= a.xmax || y >= a.ymax || x <= a.xmin || y <= a.ymin) oob = 1;
+        }
+    }
+    // one atomic per wave
+    unsigned long long bn = __ballot(nanc), bo = __ballot(oob);
+    if ((threadIdx.x % kWave) == 0) {
+        if (bn) atomicAdd(&a.flags[FLAG_NAN], (int)__popcll(bn));
+        if (bo) atomicAdd(&a.flags[FLAG_OOB], (int)__popcll(bo));
+    }
+}
+
+int st_prepdata(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PREP);
+    const vox_params& p = c->p;
+    PrepArgs a;
+    a.n = c->tp; a.dim = p.dim;
+    a.first = (p.test_kind == VOX_TEST_DROP) ? c->tp - c->fp - 1 : c->tp - c->fp;
+    a.xmin = p.xmin; a.xmax = p.xmax; a.ymin = p.ymin; a.ymax = p.ymax; a.zmin = p.zmin; a.zmax = p.zmax;
+    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+    a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW]; a.pn = c->f[VOX_F_PNEW];
+    a.u = c->f[VOX_F_U]; a.v = c->f[VOX_F_V]; a.w = c->f[VOX_F_W]; a.p = c->f[VOX_F_P];
+    a.flags = c->dflags;
+    VOX_TRY(fill_i32(c, c->dflags + FLAG_NAN, 0, 2));
+    k_prepdata<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+}  // namespace vox
