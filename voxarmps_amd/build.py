@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvoxarmps.so")
-SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_ppe.hip", "vox_io.cpp"]
+SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_ppe.hip", "vox_cg.hip",
+           "vox_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [

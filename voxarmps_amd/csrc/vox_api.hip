@@ -275,8 +275,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         if (w == 1 && !c->two_lists) break;
         DevList& L = c->list[w];
         if ((rc = dalloc(&L.cnt, n)) || (rc = dalloc(&L.cell, n)) || (rc = dalloc(&L.order_raw, n)) ||
-            (rc = dalloc(&L.order, n)) || (rc = dalloc(&L.xs, n)) || (rc = dalloc(&L.ys, n)) ||
-            (rc = dalloc(&L.zs, n)))
+            (rc = dalloc(&L.rec, n)))
             return bail(rc);
         double re = (w == 0) ? c->p.re_s : c->p.re_l;
         int want = (w == 0 ? c->p.cap_s : c->p.cap_l);
@@ -285,7 +284,9 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
     }
     for (int k = 0; k < 6; ++k)
         if ((rc = dalloc(&c->sps_s[k], n))) return bail(rc);
-    if ((rc = dalloc(&c->scratch_d, n))) return bail(rc);
+    if ((rc = dalloc(&c->scratch_d, n)) || (rc = dalloc(&c->pred_rec, n)) ||
+        (rc = dalloc(&c->pos_rec, n)) || (rc = dalloc(&c->blockmax, (size_t)grid_for(n))))
+        return bail(rc);
     if ((rc = dalloc(&c->uidx, n)) || (rc = dalloc(&c->urow, n)) || (rc = dalloc(&c->uflag, n)) ||
         (rc = dalloc(&c->rowlen, n)) || (rc = dalloc(&c->adiag, n)) || (rc = dalloc(&c->rhs, n)) ||
         (rc = dalloc(&c->cg_x, n)) || (rc = dalloc(&c->cg_r, n)) || (rc = dalloc(&c->cg_p, n)) ||
@@ -295,7 +296,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
         (rc = dalloc(&c->coll_logn, 1)))
         return bail(rc);
-    if ((rc = dalloc(&c->partials, 4 * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
+    if ((rc = dalloc(&c->partials, 8 * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
         (rc = dalloc(&c->dflags, FLAG_COUNT)))
         return bail(rc);
     if (hipHostMalloc((void**)&c->hflags, sizeof(int32_t) * FLAG_COUNT, hipHostMallocDefault) != hipSuccess ||
@@ -317,10 +318,10 @@ int vox_destroy(vox_ctx* c)
     for (int w = 0; w < 2; ++w) {
         DevList& L = c->list[w];
         dfree(L.cnt); dfree(L.ids); dfree(L.cell); dfree(L.cell_start); dfree(L.cell_fill);
-        dfree(L.order_raw); dfree(L.order); dfree(L.xs); dfree(L.ys); dfree(L.zs);
+        dfree(L.order_raw); dfree(L.rec);
     }
     for (int k = 0; k < 6; ++k) dfree(c->sps_s[k]);
-    dfree(c->scratch_d);
+    dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pos_rec); dfree(c->blockmax);
     dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->rowlen); dfree(c->acol);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q);
@@ -565,10 +566,10 @@ int vox_get_neighbors(vox_ctx* c, int which, int32_t* counts, int32_t* ids, int 
     if (stride < mx) return fail(VOX_ERR_OVERFLOW, "stride smaller than the longest row (" +
                                                        std::to_string(mx) + ")");
     if (mx == 0) return VOX_OK;
-    std::vector<int32_t> sm((size_t)mx * n);
-    VOX_HIP(hipMemcpy(sm.data(), L.ids, sizeof(int32_t) * (size_t)mx * n, hipMemcpyDeviceToHost));
+    std::vector<int32_t> sm((size_t)sl_size((long)n, L.cap));
+    VOX_HIP(hipMemcpy(sm.data(), L.ids, sizeof(int32_t) * sm.size(), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i)
-        for (int l = 0; l < cnt[i]; ++l) ids[i * stride + l] = sm[(size_t)l * n + i];
+        for (int l = 0; l < cnt[i]; ++l) ids[i * stride + l] = sm[sl_index((long)i, l, L.cap)];
     return VOX_OK;
 }
 

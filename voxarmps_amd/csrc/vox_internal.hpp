@@ -18,6 +18,16 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
 
+// Gather records: everything a neighbour loop reads about particle j in one
+// 32- or 64-byte aligned record (one cache line) instead of one line per array.
+struct __align__(32) PosRec {
+    double x, y, z;
+    int32_t id, aux;
+};
+struct __align__(64) PredRec {
+    double x, y, z, u, v, w, pn, meu;
+};
+
 // ---- error plumbing ----------------------------------------------------------
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
@@ -89,18 +99,27 @@ struct DevList {
     int32_t* cell_start = nullptr; // tnc + 1
     int32_t* cell_fill = nullptr;  // tnc
     int32_t* order_raw = nullptr;  // tp (atomic scatter result)
-    int32_t* order = nullptr;      // tp, ids sorted by (cell, id)
-    double* xs = nullptr;          // tp positions in cell order
-    double* ys = nullptr;
-    double* zs = nullptr;
+    PosRec* rec = nullptr;         // tp: positions + id sorted by (cell, id)
     long tnc_alloc = 0;
 };
+
+// Sliced row storage (SELL-64): rows are grouped in slices of 64 (one
+// wavefront); inside a slice entry l of row i sits at
+//   ((i / 64) * cap + l) * 64 + i % 64,
+// so a wave reading slot l of its 64 rows touches one contiguous 256-byte
+// line set, and a row's successive slots are 64 entries apart (the whole row
+// of a slice spans cap*256 bytes instead of cap pages).
+__host__ __device__ __forceinline__ long sl_index(long i, int l, int cap)
+{
+    return ((i >> 6) * (long)cap + l) * 64 + (i & 63);
+}
+inline long sl_size(long n, int cap) { return ((n + 63) / 64) * 64 * (long)cap; }
 
 // Lightweight by-value view of a list for kernels.
 struct ListView {
     const int32_t* __restrict__ cnt;
     const int32_t* __restrict__ ids;
-    long stride;             // == tp
+    int cap;
 };
 
 struct ProfRec { int kclass; hipEvent_t a, b; };
@@ -126,6 +145,8 @@ struct vox_ctx {
     // strain-rate scratch of SPS (s11 s12 s22 s13 s23 s33)
     double* sps_s[6] = {};
     double* scratch_d = nullptr;   // tp doubles
+    vox::PredRec* pred_rec = nullptr;  // tp gather records of the predictor
+    vox::PosRec* pos_rec = nullptr;    // tp gather records (positions + two ints)
 
     // PPE (rows are unknowns r = uidx[i]; slot-major with stride tp)
     int32_t* uidx = nullptr;       // tp: particle -> unknown or -1
@@ -161,6 +182,7 @@ struct vox_ctx {
     // scan scratch
     int32_t* scan_blocks = nullptr;
     long scan_blocks_cap = 0;
+    int32_t* blockmax = nullptr;   // one slot per 256-particle block
 
     // reductions
     double* partials = nullptr;    // 4 * kMaxPartials
@@ -203,7 +225,7 @@ struct ProfScope {
 inline ListView view(const vox_ctx* c, int which)
 {
     const DevList& L = (which == VOX_LIST_L && c->two_lists) ? c->list[1] : c->list[0];
-    return ListView{L.cnt, L.ids, (long)c->tp};
+    return ListView{L.cnt, L.ids, L.cap};
 }
 inline const DevList& list_of(const vox_ctx* c, int which)
 {
@@ -245,6 +267,18 @@ int ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32_t* 
 
 // ---- device helpers ------------------------------------------------------------------
 namespace vox {
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// This remap hands every XCD one contiguous range of logical blocks, so the
+// spatially adjacent particles of neighbouring blocks share an L2.
+__device__ __forceinline__ int xcd_block()
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+__device__ __forceinline__ int xcd_tid() { return xcd_block() * blockDim.x + threadIdx.x; }
+
 
 // Weight function W of functions.cu:32-70 (ktype 2 and 6 on the hot path).
 __device__ __forceinline__ double wfun(double r, int ktype, double re)

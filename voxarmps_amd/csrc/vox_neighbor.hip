@@ -16,7 +16,7 @@
 //
 // Pipeline per list: key+count (atomic histogram) -> exclusive scan (cell
 // start) -> atomic scatter -> per-cell rank sort by id (one wave per cell,
-// LDS tile) with the positions copied into cell order -> per-particle stencil
+// LDS tile) with position records copied into cell order -> per-particle stencil
 // walk writing slot-major rows ids[l*tp + i].
 #include "vox_internal.hpp"
 
@@ -216,9 +216,8 @@ constexpr int kSortTile = 512;   // ids per wave held in LDS
 
 __global__ __launch_bounds__(kBlock) void k_cell_sort(
     long tnc, int dim, const int32_t* __restrict__ start, const int32_t* __restrict__ order_raw,
-    int32_t* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, double* __restrict__ xs, double* __restrict__ ys,
-    double* __restrict__ zs)
+    PosRec* __restrict__ rec, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z)
 {
     __shared__ int32_t tile[kBlock / kWave][kSortTile];
     int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -240,10 +239,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
             rank += (o < id);
         }
         int pos = s + rank;
-        order[pos] = id;
-        xs[pos] = x[id];
-        ys[pos] = y[id];
-        if (dim == 3) zs[pos] = z[id];
+        PosRec rr;
+        rr.x = x[id];
+        rr.y = y[id];
+        rr.z = (dim == 3) ? z[id] : 0.0;
+        rr.id = id;
+        rr.aux = 0;
+        rec[pos] = rr;
     }
 }
 
@@ -257,12 +259,11 @@ __device__ __forceinline__ int wave_max_i32(int v)
 __global__ __launch_bounds__(kBlock) void k_build_list(
     Grid g, int n, double R, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int32_t* __restrict__ cell,
-    const int32_t* __restrict__ start, const int32_t* __restrict__ order,
-    const double* __restrict__ xs, const double* __restrict__ ys, const double* __restrict__ zs,
+    const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
     int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
-    int flag_overflow, int flag_max)
+    int flag_overflow, int flag_max, int32_t* __restrict__ blockmax)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_tid();
     int k = 0;
     if (i < n) {
         const int32_t key = cell[i];
@@ -277,38 +278,65 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
         const double xi = x[i], yi = y[i], zi = (g.dim == 3) ? z[i] : 0.0;
         const double R2 = R * R;
         const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
-        const long stride = n;
+        const long rb = sl_index(i, 0, cap);
         for (int row = m1; row <= m2; ++row)
             for (int col = m3; col <= m4; ++col)
                 for (int dep = m5; dep <= m6; ++dep) {
                     int cn = (ic - 1 + row) + (jc - 1 + col) * g.ncx + (kc - 1 + dep) * ncxy;
-                    int s = start[cn], e = start[cn + 1];
-                    for (int jj = s; jj < e; ++jj) {
-                        double dx = xs[jj] - xi;
-                        double dy = ys[jj] - yi;
-                        double d2;
-                        if (g.dim == 3) {
-                            double dz = zs[jj] - zi;
-                            d2 = dx * dx + dy * dy + dz * dz;
-                        } else {
-                            d2 = dx * dx + dy * dy;
+                    const int s = start[cn], e = start[cn + 1];
+                    // chunks of 8 candidates: loads first, then the ordered
+                    // accept/store pass (stores would serialise later loads)
+                    constexpr int U = 8;
+                    for (int j0 = s; j0 < e; j0 += U) {
+                        PosRec cr[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) cr[u] = rec[(j0 + u < e) ? j0 + u : s];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) {
+                            // branch-free test (the loads above stay unconditional)
+                            // z is 0 in 2D records: dx^2+dy^2+0 equals the 2D sum
+                            const PosRec cu = cr[u];
+                            const double dx = cu.x - xi, dy = cu.y - yi, dz = cu.z - zi;
+                            const double d2 = dx * dx + dy * dy + dz * dz;
+                            bool acc = d2 < lo;
+                            if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
+                                acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                            acc = acc && (j0 + u < e) && (g.dim == 3 || cu.id != i);
+                            if (acc) {
+                                if (k < cap) ids[rb + 64L * k] = cu.id;
+                                ++k;
+                            }
                         }
-                        bool acc;
-                        if (d2 < lo) acc = true;
-                        else if (d2 > hi) acc = false;
-                        else acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
-                        if (!acc) continue;
-                        int j = order[jj];
-                        if (g.dim != 3 && j == i) continue;
-                        if (k < cap) ids[(long)k * stride + i] = j;
-                        ++k;
                     }
                 }
         cnt[i] = k;
         if (k > cap) atomicAdd(&flags[flag_overflow], 1);
     }
+    // longest row: wave max -> block max -> one slot per block (no atomics)
+    __shared__ int wmax[kBlock / kWave];
     int m = wave_max_i32(k);
-    if ((threadIdx.x % kWave) == 0) atomicMax(&flags[flag_max], m);
+    if ((threadIdx.x % kWave) == 0) wmax[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+        for (int w = 1; w < kBlock / kWave; ++w) b = max(b, wmax[w]);
+        blockmax[blockIdx.x] = b;
+    }
+    (void)flag_max;
+}
+
+__global__ void k_reduce_max(const int32_t* __restrict__ v, int n, int32_t* __restrict__ out)
+{
+    __shared__ int lds[kBlock];
+    int m = 0;
+    for (int k = threadIdx.x; k < n; k += kBlock) m = max(m, (int)v[k]);
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) lds[threadIdx.x] = max(lds[threadIdx.x], lds[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = lds[0];
 }
 
 // ---- host side --------------------------------------------------------------------------
@@ -318,7 +346,7 @@ int ensure_list_storage(vox_ctx* c, int which, int cap)
     if (cap <= L.cap && L.ids) return VOX_OK;
     if (L.ids) (void)hipFree(L.ids);
     L.ids = nullptr;
-    VOX_HIP(hipMalloc(&L.ids, sizeof(int32_t) * (size_t)cap * c->tp));
+    VOX_HIP(hipMalloc(&L.ids, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
     L.cap = cap;
     return VOX_OK;
 }
@@ -357,9 +385,8 @@ static int build_one(vox_ctx* c, int which, double re)
                                                            L.order_raw);
         long nb = (L.g.tnc + (kBlock / kWave) - 1) / (kBlock / kWave);
         k_cell_sort<<<(int)nb, kBlock, 0, c->stream>>>(L.g.tnc, c->p.dim, L.cell_start,
-                                                        L.order_raw, L.order, c->f[VOX_F_X],
-                                                        c->f[VOX_F_Y], c->f[VOX_F_Z], L.xs, L.ys,
-                                                        L.zs);
+                                                        L.order_raw, L.rec, c->f[VOX_F_X],
+                                                        c->f[VOX_F_Y], c->f[VOX_F_Z]);
         VOX_HIP(hipGetLastError());
     }
     double R = (c->p.dim == 3) ? re + c->p.delta : re;
@@ -367,10 +394,11 @@ static int build_one(vox_ctx* c, int which, double re)
         {
             ProfScope ps(c, KC_LIST);
             VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
-            VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-            k_build_list<<<grid_for(n), kBlock, 0, c->stream>>>(
+            const int nb = grid_for(n);
+            k_build_list<<<nb, kBlock, 0, c->stream>>>(
                 L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
-                L.order, L.xs, L.ys, L.zs, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+                L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
+            k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
             VOX_HIP(hipGetLastError());
         }
         VOX_TRY(read_flags(c));

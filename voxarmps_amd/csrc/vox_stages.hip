@@ -9,7 +9,7 @@
 
 namespace vox {
 
-#define VOX_ROW_J(L, l, i) ((L).ids[(long)(l) * (L).stride + (i)])
+#define VOX_ROW_J(L, l, i) ((L).ids[sl_index((i), (l), (L).cap)])
 
 __device__ __forceinline__ double pdist(int dim, const double* __restrict__ x,
                                         const double* __restrict__ y,
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, doubl
                                                 const double* __restrict__ p,
                                                 double* __restrict__ pnew)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= n) return;
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     double sum = 0.0;
@@ -175,7 +175,7 @@ struct SpsArgs {
 
 __global__ __launch_bounds__(kBlock) void k_sps1(SpsArgs a)
 {
-    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    int i = a.first + xcd_tid();
     if (i >= a.n) return;
     Strain S = strain_row(i, a.dim, a.ktype, a.re, a.n0, a.dmin, true, a.L, a.x, a.y, a.z, a.u,
                           a.v, a.w);
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_sps1(SpsArgs a)
 
 __global__ __launch_bounds__(kBlock) void k_sps2(SpsArgs a)
 {
-    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    int i = a.first + xcd_tid();
     if (i >= a.n) return;
     double xi = a.x[i], yi = a.y[i], zi = (a.dim == 3) ? a.z[i] : 0.0, ni = a.neut[i];
     double s1 = 0, s2 = 0, s3 = 0;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_vfraction(int n, int dim, int ktype,
                                                       const double* __restrict__ z,
                                                       double* __restrict__ C)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= n) return;
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     double s1 = 0, s2 = 0;
@@ -300,7 +300,7 @@ struct HbArgs {
 
 __global__ __launch_bounds__(kBlock) void k_visc_hb(HbArgs a)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= a.n) return;
     if (a.pt[i] == 1) { a.meu[i] = a.m1; return; }
     Strain S = strain_row(i, a.dim, a.ktype, a.re, a.n0, 0.0, false, a.L, a.x, a.y, a.z, a.u, a.v,
@@ -350,11 +350,24 @@ struct PredArgs {
     const int32_t* pt;
     const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
     double *xs, *ys, *zs, *us, *vs, *ws;
+    PredRec* rec;
 };
+
+// One 64-byte record per particle with every field the predictor gathers.
+__global__ void k_pack_pred(PredArgs a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    PredRec r;
+    r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
+    r.u = a.u[i]; r.v = a.v[i]; r.w = (a.dim == 3) ? a.w[i] : 0.0;
+    r.pn = a.pn[i]; r.meu = a.meu[i];
+    a.rec[i] = r;
+}
 
 __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= a.n) return;
     const int dim = a.dim;
     const int ti = a.pt[i];
@@ -363,39 +376,51 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
         if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
         return;
     }
-    double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
-    double ui = a.u[i], vi = a.v[i], wi = (dim == 3) ? a.w[i] : 0.0;
-    double mi = a.meu[i], phi_ = a.ph[i];
+    const PredRec me = a.rec[i];
+    const double xi = me.x, yi = me.y, zi = me.z, ui = me.u, vi = me.v, wi = me.w;
+    const double mi = me.meu, phi_ = a.ph[i];
     double rhoN = (ti == 1) ? a.rho1 : a.rho2;
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
-    int cnt = a.L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(a.L, l, i);
-        if (j == i) continue;
-        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
-        if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
-        else D = sqrt(dx * dx + dy * dy);
-        if (!(D > 0)) continue;
-        double NEU;
-        if (ti == a.pt[j]) NEU = mi / rhoN;
-        else {
-            double mj = a.meu[j];
-            NEU = 2 * mi * mj / (mi + mj) / rhoN;
-        }
-        double wg = wfun(D, a.ktype, a.re_s);
-        double wl = (a.re_l == a.re_s) ? wg : wfun(D, a.ktype, a.re_l);
-        double dp = a.pn[j] - phi_;
-        s1 = s1 + dp * dx * wg / D / D;
-        s2 = s2 + dp * dy * wg / D / D;
-        if (dim == 3) s3 = s3 + dp * dz * wg / D / D;
-        double du = a.u[j] - ui, dv = a.v[j] - vi;
-        s4 = s4 + wl * du * NEU;
-        s5 = s5 + wl * dv * NEU;
-        double dw = 0.0;
-        if (dim == 3) { dw = a.w[j] - wi; s6 = s6 + wl * dw * NEU; }
-        s7 = s7 + wl * du;
-        s8 = s8 + wl * dv;
-        if (dim == 3) s9 = s9 + wl * dw;
+    const int cnt = a.L.cnt[i];
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    // chunks of 4 neighbours: their index, record and type loads are issued
+    // together, then the terms are accumulated in row order
+    constexpr int U = 4;
+    for (int l0 = 0; l0 < cnt; l0 += U) {
+    int jj[U], tj[U];
+    PredRec rv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+#pragma unroll
+    for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
+#pragma unroll
+    // Branch-free per neighbour: every term is computed and a skipped entry
+    // (self, padding, coincident particle) keeps each sum through a select, so
+    // no gathered field is sunk into a branch behind a full memory wait.
+    for (int u = 0; u < U; ++u) {
+        const PredRec rj = rv[u];
+        // z, w are 0 in 2D records: the 3-term distance equals the 2D one
+        // bit for bit, and no load is left inside a dimension branch
+        const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
+        const double D = sqrt(dx * dx + dy * dy + dz * dz);
+        const bool use = (jj[u] != i) && (D > 0);
+        const double mj = rj.meu;
+        const double hm = 2 * mi * mj / (mi + mj) / rhoN;
+        const double NEU = (ti == tj[u]) ? mi / rhoN : hm;
+        const double wg = wfun(D, a.ktype, a.re_s);
+        const double wl = (a.re_l == a.re_s) ? wg : wfun(D, a.ktype, a.re_l);
+        const double dp = rj.pn - phi_;
+        const double du = rj.u - ui, dv = rj.v - vi, dw = rj.w - wi;
+        s1 = use ? s1 + dp * dx * wg / D / D : s1;
+        s2 = use ? s2 + dp * dy * wg / D / D : s2;
+        s4 = use ? s4 + wl * du * NEU : s4;
+        s5 = use ? s5 + wl * dv * NEU : s5;
+        s7 = use ? s7 + wl * du : s7;
+        s8 = use ? s8 + wl * dv : s8;
+        s3 = use ? s3 + dp * dz * wg / D / D : s3;   // z sums: only read in 3D
+        s6 = use ? s6 + wl * dw * NEU : s6;
+        s9 = use ? s9 + wl * dw : s9;
+    }
     }
     double nt = a.neut[i];
     double gc = a.relax1 * (a.gcoef_num * a.dt / a.n0s / a.rho[i]);
@@ -433,6 +458,8 @@ int st_prediction(vox_ctx* c)
     a.t1 = c->f[VOX_F_TURB1]; a.t2 = c->f[VOX_F_TURB2]; a.t3 = c->f[VOX_F_TURB3];
     a.xs = c->f[VOX_F_XSTAR]; a.ys = c->f[VOX_F_YSTAR]; a.zs = c->f[VOX_F_ZSTAR];
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
+    a.rec = c->pred_rec;
+    k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     k_prediction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
@@ -483,7 +510,7 @@ int st_clamp_pressure(vox_ctx* c)
 // ---- PHATCALC: functions.cu:207-224 -------------------------------------------------------------
 __global__ void k_phat(int n, ListView L, const double* __restrict__ pn, double* __restrict__ ph)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= n) return;
     double mn = 999999999999999.0;
     int cnt = L.cnt[i];
@@ -514,7 +541,7 @@ struct GradArgs {
 
 __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
 {
-    int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
+    int i = a.first + xcd_tid();
     if (i >= a.n) return;
     const int dim = a.dim;
     double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
@@ -640,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double r
                                                         int32_t* __restrict__ ci,
                                                         int32_t* __restrict__ cj, long cap)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = xcd_tid();
     if (i >= n) return;
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     long pos = off ? off[i] : 0;
