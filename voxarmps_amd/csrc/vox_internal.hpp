@@ -295,16 +295,6 @@ __device__ __forceinline__ double wfun(double r, int ktype, double re)
     return 0.0;
 }
 
-// Agent-scope load that bypasses the non-coherent vector L1.
-__device__ __forceinline__ double load_agent(const double* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int32_t load_agent_i(const int32_t* p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Fixed-order block sum of one double per thread (blockDim == kBlock).
 __device__ __forceinline__ double block_sum(double v, double* lds)
 {
@@ -319,42 +309,6 @@ __device__ __forceinline__ double block_sum(double v, double* lds)
     double r = lds[0];
     __syncthreads();
     return r;
-}
-
-// Deterministic grid-wide reduction of nv doubles per thread. Each block writes
-// its partials; the last block to finish sums them in block order and calls
-// fin(totals) from thread 0. Returns true in the finishing block.
-template <int NV, class Fin>
-__device__ __forceinline__ void grid_reduce(const double (&v)[NV], double* partials,
-                                            int32_t* counter, double* lds, Fin fin)
-{
-    double bs[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], lds);
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) partials[k * kMaxPartials + blockIdx.x] = bs[k];
-        __threadfence();
-        int ticket = atomicAdd(counter, 1);
-        is_last = (ticket == (int)gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double tot[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += kBlock)
-            s = s + load_agent(&partials[k * kMaxPartials + b]);
-        tot[k] = block_sum(s, lds);
-    }
-    if (threadIdx.x == 0) {
-        fin(tot);
-        *counter = 0;
-        __threadfence();
-    }
 }
 
 }  // namespace vox
