@@ -26,12 +26,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nu, nnz_total):
-    """Compulsory bytes of one q = A p launch: off-diagonal values (8 B) and
-    column ids (4 B), diagonal (8 B), row length (4 B), p read once (8 B) and
-    q written (8 B) per row; gathers of p are counted once."""
+def spmv_bytes(nu, nnz_total, vbytes=8):
+    """Compulsory bytes of one q = A p launch: off-diagonal values (vbytes)
+    and column ids (4 B), diagonal (8 B, FP64 in both modes), row length
+    (4 B), p read once and q written (vbytes each) per row; gathers of p are
+    counted once. vbytes is 8 in FP64 and 4 for the FP32 inner solves of the
+    mixed mode."""
     off = nnz_total - nu
-    return off * 12 + nu * (8 + 4 + 8 + 8)
+    return off * (4 + vbytes) + nu * (8 + 4 + 2 * vbytes)
 
 
 def cpu_baseline(sample_scale=2, steps=2):
@@ -65,6 +67,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=1, help="linear down-scaling of the workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="fp64: every kernel in FP64 (the reference's arithmetic); mixed: FP32 "
+                         "velocity stages and FP32 inner CG inside FP64 iterative refinement")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="start each PPE solve from the previous step's pressure")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -80,6 +87,10 @@ def main():
 
     case = vx.dam_break_3d_c1(scale=args.scale)
     p = vx.dam3d_params(case)
+    mixed = args.precision == "mixed"
+    if mixed:
+        p.precision = vx._abi.PREC_MIXED
+    p.warm_start = 1 if args.warm_start else 0
     sim = vx.Simulation(p, case, backend=be)
     sim.init()
     print(f"# init done: {case.tp} particles", file=sys.stderr, flush=True)
@@ -108,10 +119,10 @@ def main():
     nl = C.c_int64(0)
     be.lib.vox_prof_get(sim.ctx, b"spmv", C.byref(ms), C.byref(nl))
     avg_ms = ms.value / max(nl.value, 1)
-    bytes_launch = spmv_bytes(nu, nnz)
+    bytes_launch = spmv_bytes(nu, nnz, 4 if mixed else 8)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     per_kernel = {}
-    for k in ("list", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p",
+    for k in ("list", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p", "ir_residual",
               "collision", "pgrad", "phat", "prep", "euler", "bcon", "visc", "vfrac", "misc"):
         t = C.c_double(0.0)
         n = C.c_int64(0)
@@ -132,20 +143,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {
-            "workload": "3D dam break, plain MPS, FP64, l0=0.005 m, water 100x100x80 in "
-                        "1.6x0.6x0.405 m tank, 1 wall + 2 dummy layers",
+            "workload": "3D dam break, plain MPS, " + ("mixed FP32/FP64" if mixed else "FP64") +
+                        ", l0=0.005 m, water 100x100x80 in 1.6x0.6x0.405 m tank, "
+                        "1 wall + 2 dummy layers",
             "particles": case.tp,
             "fluid": case.fp,
             "re_s": "2.1 l0", "re_l": "4.0 l0", "dt": p.dt,
-            "ppe": f"Jacobi-PCG FP64 to rel-res {p.cg_tol:g}",
+            "ppe": (f"FP64 iterative refinement around FP32 Jacobi-PCG (eta {p.ir_eta:g}) to "
+                    f"rel-res {p.cg_tol:g}" if mixed else f"Jacobi-PCG FP64 to rel-res {p.cg_tol:g}")
+                   + (", warm start" if args.warm_start else ", cold start"),
             "cg_iters_mean": sum(iters) / len(iters),
             "parallelism": "1 GPU",
         },
         "roofline": {
-            "kernel": "k_cg_spmv (PPE q = A p + p.q)",
+            "kernel": "k_cg_spmv (PPE q = A p + p.q)" + (", FP32" if mixed else ", FP64"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

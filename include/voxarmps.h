@@ -111,6 +111,12 @@ typedef struct vox_params {
     int32_t cg_check;     /* iterations between host convergence checks         */
     int32_t cap_s;        /* neighbour row capacity for list S (0 = auto)        */
     int32_t cap_l;        /* neighbour row capacity for list L (0 = auto)        */
+    double  ir_eta;       /* mixed precision: relative tolerance of each FP32
+                             inner solve of the FP64 refinement (0 = 1e-4)       */
+    int32_t ir_maxouter;  /* refinement steps cap (mixed precision, warm start)  */
+    int32_t warm_start;   /* 1: the PPE solve starts from each unknown's pressure
+                             of the previous step (field P) and refines in FP64;
+                             0: from zero, as the reference (functions.cu:2503) */
 } vox_params;
 
 /* PPE solve report */

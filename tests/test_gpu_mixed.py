@@ -1,0 +1,191 @@
+"""Mixed precision and warm-started PPE solves on the GPU, through the C ABI.
+
+Neither mode has a counterpart in the reference, so the oracle's FP64 stages
+and the mathematics pin them instead:
+* a refined solve (FP32 inner CG inside FP64 iterative refinement, or an FP64
+  solve started from the previous pressure) must meet ||b - A x|| <= tol ||b||
+  on the exported FP64 system, with the residual recomputed here in numpy;
+* its solution must agree with the cold FP64 CG solution of the same system
+  within the bound the two residuals allow;
+* the FP32 predictor and pressure-gradient stages must agree with the oracle's
+  FP64 stages from the same state to FP32 accuracy (tolerance stated per test);
+* a short mixed-precision run must stay close to the FP64 run.
+"""
+import numpy as np
+import pytest
+
+import voxarmps_amd as vx
+
+pytestmark = pytest.mark.gpu
+
+
+def _case3d(seed=None):
+    """Small 3D dam break: 16 x 16 x 12 water in a 48 x 20 x 13 tank, l0 5 mm.
+    The lattice (seed None) is the physical start: n0 = max n is then the
+    lattice value. With jitter, max n exceeds the lattice value and almost every
+    particle falls below the free-surface threshold, which suits stage tests
+    but leaves the PPE nearly empty."""
+    c = vx.dam_break(3, 0.005, (16, 16, 12), (48, 20, 13))
+    return c if seed is None else vx.jitter(c, 0.1 * 0.005, seed=seed)
+
+
+def _to_matrix(sim):
+    """The stage sequence of one time step up to the PPE assembly."""
+    sim.PNUM()
+    sim.NEIGHBOR()
+    sim.V_FRACTION()
+    sim.VISCOSITY()
+    sim.PREDICTION()
+    sim.PNUMSTAR()
+    sim.BCON()
+    sim.MATRIX()
+
+
+def _residual(A, x):
+    n = A["rhs"].shape[0]
+    rowptr, cols, vals = A["rowptr"], A["cols"], A["vals"]
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    ax = np.zeros(n)
+    np.add.at(ax, rows, vals * x[cols])
+    return np.linalg.norm(A["rhs"] - ax) / np.linalg.norm(A["rhs"])
+
+
+@pytest.mark.parametrize("precision,warm", [("mixed", 0), ("mixed", 1), ("fp64", 1)])
+def test_refined_solve_meets_fp64_residual(hip, precision, warm):
+    case = _case3d()
+    p = vx.dam3d_params(case, tol=1e-8)
+    sim = vx.Simulation(p, case, backend=hip)
+    sim.init()
+    sim.step(4)                        # nonzero previous pressure for the warm start
+    _to_matrix(sim)
+    A = sim.ppe()
+    cold = sim.CGM()
+    x_cold = sim.get("PNEW")[A["ids"]]
+    assert cold.converged and _residual(A, x_cold) <= 1.01e-8
+    assert A["rhs"].shape[0] > 3000 and np.linalg.norm(A["rhs"]) > 0
+    q = p.copy()
+    q.precision = vx._abi.PREC_MIXED if precision == "mixed" else vx._abi.PREC_FP64
+    q.warm_start = warm
+    sim.set_params(q)
+    sim.MATRIX()                       # same FP64 system (plus the FP32 copy in mixed mode)
+    A2 = sim.ppe()
+    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+        assert np.array_equal(A[k], A2[k]), k
+    info = sim.CGM()
+    x = sim.get("PNEW")[A["ids"]]
+    rel = _residual(A, x)
+    print(f"{precision} warm={warm}: iters {info.iterations} (cold FP64 {cold.iterations}), "
+          f"relres gpu {info.relres:.3e} numpy {rel:.3e}")
+    assert info.converged
+    assert info.relres <= 1e-8
+    assert rel <= 1.01e-8
+    # both solutions have relative residual <= 1e-8 on a matrix whose Jacobi-
+    # scaled condition number is small (diagonal shift 1e-7/dt^2), so they agree
+    # far inside 1e-6 of the pressure scale
+    assert np.max(np.abs(x - x_cold)) <= 1e-6 * np.max(np.abs(x_cold))
+    # (a warm start does not lower the count on these systems: the previous
+    # pressure is no closer to the new solution than zero is, relative to b)
+    assert info.iterations <= 2 * cold.iterations + 2
+    sim.close()
+
+
+def _hydrostatic(sim, case, p):
+    h = case.y.max()
+    pr = p.rho1 * 9.81 * np.maximum(h - case.y, 0.0)
+    sim.set("PNEW", pr)
+    sim.set("P", pr)
+    return pr
+
+
+def test_fp32_velocity_stages_track_fp64(hip, oracle):
+    """Predictor and pressure gradient in FP32 against the oracle's FP64
+    stages from one state (hydrostatic pressure, random velocities of 0.3 m/s
+    scale). Predictor bound: 1e-5 of each velocity's magnitude, and dt times
+    that for x*. Gradient bound: the FP32 rounding bound of its sum, below."""
+    case = _case3d(5)
+    p = vx.dam3d_params(case, tol=1e-10)
+    q = p.copy()
+    q.precision = vx._abi.PREC_MIXED
+    g = vx.Simulation(q, case, backend=hip)
+    c = vx.Simulation(p, case, backend=oracle)
+    for s in (g, c):
+        s.init()
+    rng = np.random.default_rng(8)
+    for name in ("U", "V", "W", "UNEW", "VNEW", "WNEW"):
+        v = rng.normal(0, 0.3, case.tp)
+        g.set(name, v)
+        c.set(name, v)
+    for s in (g, c):
+        _hydrostatic(s, case, p)
+        s.PNUM()
+        s.NEIGHBOR()
+        s.V_FRACTION()
+        s.VISCOSITY()
+        s.PHATCALC()
+        s.PREDICTION()
+    for n in ("USTAR", "VSTAR", "WSTAR"):
+        a, b = g.get(n), c.get(n)
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"PREDICTION {n}: max rel err {err:.2e}")
+        assert err <= 1e-5, n
+    for n in ("XSTAR", "YSTAR", "ZSTAR"):
+        a, b = g.get(n), c.get(n)
+        assert np.max(np.abs(a - b)) <= p.dt * 1e-5 * 3.0, n
+    # gradient from identical x*, u*, p-hat
+    for n in ("XSTAR", "YSTAR", "ZSTAR", "USTAR", "VSTAR", "WSTAR", "PHAT"):
+        g.set(n, c.get(n))
+    g.PRESSGRAD()
+    c.PRESSGRAD()
+    # FP32 rounding bound of the gradient sum, per particle and axis:
+    # 2^-18 (32 FP32 ulps) of sum_j |p_j + p_i - (phat_i + phat_j)| |dx_k| W / D^2
+    # (KH = 1 form), times the coefficient relaxp * 2 dt / (n0 rho_i), plus the
+    # FP32 rounding of u* and of the result.
+    cnt, ids = c.neighbors(vx.LIST_S)
+    tp = case.tp
+    X = np.stack([c.get("XSTAR"), c.get("YSTAR"), c.get("ZSTAR")], 1)
+    pn, ph = c.get("PNEW"), c.get("PHAT")
+    valid = (np.arange(ids.shape[1])[None, :] < cnt[:, None]) & (ids != np.arange(tp)[:, None])
+    J = np.where(valid, ids, 0)
+    d = X[J] - X[:, None, :]
+    D = np.sqrt((d ** 2).sum(-1))
+    Ds = np.where(valid, D, 1.0)
+    qq = Ds / p.re_s
+    w = np.where(valid & (qq < 1.0), 1.0 / qq - 1.0, 0.0)
+    dp = np.abs(pn[J] + pn[:, None] - (ph[:, None] + ph[J]))
+    coef = p.relaxp * (2.0 * p.dt / c.n0()[0] / c.get("RHO"))
+    f = case.ptype > 0
+    for k, (n, u0) in enumerate((("UNEW", "USTAR"), ("VNEW", "VSTAR"), ("WNEW", "WSTAR"))):
+        B = (dp * np.abs(d[..., k]) * w / Ds ** 2).sum(1)
+        a, b, us = g.get(n), c.get(n), c.get(u0)
+        bound = coef * B * 2.0 ** -18 + 2.0 ** -23 * (np.abs(us) + np.abs(b))
+        err = np.abs(a - b)
+        worst = np.max((err / bound)[f])
+        print(f"PRESSGRAD {n}: max error / bound {worst:.3f}, max abs error {err[f].max():.2e}")
+        assert np.all(err[f] <= bound[f]), n
+    g.close()
+    c.close()
+
+
+def test_mixed_run_tracks_fp64(hip):
+    """Ten steps of the small 3D dam break in mixed precision (with warm
+    start) against the FP64 run: positions within 1e-7 m and pressure within
+    1e-3 of its scale (L2)."""
+    case = _case3d()
+    p = vx.dam3d_params(case, tol=1e-8)
+    q = p.copy()
+    q.precision = vx._abi.PREC_MIXED
+    q.warm_start = 1
+    a = vx.Simulation(p, case, backend=hip)
+    b = vx.Simulation(q, case, backend=hip)
+    for s in (a, b):
+        s.init()
+        s.step(10)
+    dx = max(np.max(np.abs(a.get(n) - b.get(n))) for n in ("X", "Y", "Z"))
+    pa, pb = a.get("P"), b.get("P")
+    perr = np.linalg.norm(pa - pb) / np.linalg.norm(pa)
+    print(f"10 steps: max |dx| {dx:.2e} m, pressure L2 rel {perr:.2e}")
+    assert dx <= 1e-7
+    assert perr <= 1e-3
+    assert b.status().last_relres <= 1e-8
+    a.close()
+    b.close()

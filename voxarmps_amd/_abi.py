@@ -53,6 +53,7 @@ class VoxParams(C.Structure):
         ("test_kind", _i32), ("coef2d", _i32), ("precision", _i32), ("solver", _i32),
         ("cg_tol", _f64),
         ("cg_maxit", _i32), ("cg_check", _i32), ("cap_s", _i32), ("cap_l", _i32),
+        ("ir_eta", _f64), ("ir_maxouter", _i32), ("warm_start", _i32),
     ]
 
     def copy(self):

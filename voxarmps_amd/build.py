@@ -1,8 +1,9 @@
 """Build libvoxarmps.so in-tree with hipcc for gfx950.
 
 Every source is compiled with -ffp-contract=off (no FMA contraction, so FP64
-per-particle sums follow the reference's operation order exactly) and with
-packed-FP32 instructions disabled.
+per-particle sums follow the reference's operation order exactly), except the
+FP32 stages of the mixed-precision mode, and with packed-FP32 instructions
+disabled.
 """
 import concurrent.futures as cf
 import os
@@ -13,8 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvoxarmps.so")
-SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_ppe.hip", "vox_cg.hip",
-           "vox_io.cpp"]
+SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_stages32.hip", "vox_ppe.hip",
+           "vox_cg.hip", "vox_io.cpp"]
+# The FP32 stages of the mixed-precision mode have no bit-exact counterpart, so
+# they let the compiler fuse multiply-adds (the last -ffp-contract wins).
+EXTRA = {"vox_stages32.hip": ["-ffp-contract=fast"]}
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 FLAGS = [
@@ -26,7 +30,7 @@ FLAGS = [
 
 def _compile(src):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -22,7 +22,8 @@ int fail(int code, const std::string& msg)
 
 static const char* kKNames[KC_COUNT] = {
     "cell", "sort", "list", "pnd", "sps", "vfrac", "visc", "pred", "bcon", "assemble",
-    "cg_init", "spmv", "cg_update", "cg_p", "phat", "pgrad", "euler", "collision", "prep", "misc"};
+    "cg_init", "spmv", "cg_update", "cg_p", "phat", "pgrad", "euler", "collision", "prep", "misc",
+    "ir_residual"};
 
 const char* kclass_name(int k) { return (k >= 0 && k < KC_COUNT) ? kKNames[k] : "?"; }
 int kclass_from_name(const char* name)
@@ -204,6 +205,9 @@ int vox_params_default(vox_params* p, int dim, double dl)
     p->cg_tol = 1e-9;
     p->cg_maxit = 5000;
     p->cg_check = 1;
+    p->ir_eta = 1e-4;
+    p->ir_maxouter = 8;
+    p->warm_start = 0;
     return VOX_OK;
 }
 
@@ -238,8 +242,12 @@ static int validate(const vox_params* p)
     if (!(p->dt > 0)) return fail(VOX_ERR_ARG, "dt must be positive");
     if (!(p->xmax > p->xmin) || !(p->ymax > p->ymin) || (p->dim == 3 && !(p->zmax > p->zmin)))
         return fail(VOX_ERR_ARG, "grid box is empty: call vox_params_box");
-    if (p->precision != VOX_PREC_FP64) return fail(VOX_ERR_NOLIB, "only FP64 precision is built");
+    if (p->precision != VOX_PREC_FP64 && p->precision != VOX_PREC_MIXED)
+        return fail(VOX_ERR_NOLIB, "unknown precision mode");
     if (p->solver != VOX_SOLVER_JACOBI_CG) return fail(VOX_ERR_NOLIB, "unknown PPE solver");
+    if (!(p->ir_eta >= 0.0 && p->ir_eta < 1.0) || p->ir_maxouter < 1 ||
+        (p->warm_start != 0 && p->warm_start != 1))
+        return fail(VOX_ERR_ARG, "ir_eta must be in [0, 1), ir_maxouter >= 1, warm_start 0 or 1");
     if (p->ktype != 2 && p->ktype != 6 && p->ktype != 1 && p->ktype != 5)
         return fail(VOX_ERR_ARG, "unsupported kernel type");
     return VOX_OK;
@@ -290,7 +298,9 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
     if ((rc = dalloc(&c->uidx, n)) || (rc = dalloc(&c->urow, n)) || (rc = dalloc(&c->uflag, n)) ||
         (rc = dalloc(&c->rowlen, n)) || (rc = dalloc(&c->adiag, n)) || (rc = dalloc(&c->rhs, n)) ||
         (rc = dalloc(&c->cg_x, n)) || (rc = dalloc(&c->cg_r, n)) || (rc = dalloc(&c->cg_p, n)) ||
-        (rc = dalloc(&c->cg_q, n)))
+        (rc = dalloc(&c->cg_q, n)) || (rc = dalloc(&c->cg_x32, n)) || (rc = dalloc(&c->cg_r32, n)) ||
+        (rc = dalloc(&c->cg_p32, n)) || (rc = dalloc(&c->cg_q32, n)) || (rc = dalloc(&c->cg_d, n)) ||
+        (rc = dalloc(&c->cg_e, n)))
         return bail(rc);
     if ((rc = dalloc(&c->coll_cnt, n)) || (rc = dalloc(&c->coll_off, n + 1)) ||
         (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
@@ -324,7 +334,8 @@ int vox_destroy(vox_ctx* c)
     dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pos_rec); dfree(c->blockmax);
     dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->rowlen); dfree(c->acol);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs);
-    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q);
+    dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
+    dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
     dfree(c->coll_stamp); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
     dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);

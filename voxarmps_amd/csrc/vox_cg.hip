@@ -1,8 +1,9 @@
-// vox_cg.hip — FP64 Jacobi-preconditioned CG for the assembled PPE on gfx950.
+// vox_cg.hip — Jacobi-preconditioned CG for the assembled PPE on gfx950, in
+// FP64 or as FP64 iterative refinement around an FP32 inner solve.
 //
 // Replaces INCDECOM + CGM (functions.cu:1139-1200, 506-716) and the GPU loop of
 // main.cu:1159-1261 (which synchronised with the host and used FP32 atomics for
-// every dot product). One iteration is three kernels on a fixed grid:
+// every dot product). One CG iteration is three kernels on a fixed grid:
 //   K1 q = A p                         partial p.q per block
 //   K2 alpha = rz/p.q; x += alpha p; r -= alpha q
 //                                      partials r.r and r.(r/d) per block
@@ -14,7 +15,15 @@
 // k&1 while its blocks read slot (k-1)&1). The stop flag stays on the device;
 // kernels launched after convergence return at entry.
 //
+// The kernels are templated on the vector type VT and the matrix value type
+// MT; dot products always accumulate in FP64. VT = MT = double is the FP64
+// solver. In mixed precision the outer loop keeps x, the residual b - A x and
+// its norm in FP64 on the FP64 matrix, and each correction solves A d = r with
+// FP32 matrix values and FP32 vectors (FP64 dots) to a relative eta.
+//
 // The matrix is sliced ELL (SELL-64); the product runs one thread per row.
+#include <type_traits>
+
 #include "vox_internal.hpp"
 
 namespace vox {
@@ -22,28 +31,31 @@ namespace vox {
 constexpr int kCgBlocks = 2048;   // 8192 waves: 8 per SIMD on 256 CUs
 
 // partial slots (kMaxPartials doubles each)
-enum { P_BB = 0, P_RZ0, P_NNZ, P_PQ, P_RR, P_RZ, P_COUNT };
-#define PSLOT(a, s) ((a).part + (long)(s) * kMaxPartials)
+enum { P_BB = 0, P_RZ0, P_NNZ, P_PQ, P_RR, P_RZ, P_RES, P_RHS, P_COUNT };
 // scalar slots
-enum { S_BNORM = 0, S_RZ0, S_RZ1, S_RELRES, S_NNZ, S_COUNT_ };
+enum { S_BNORM = 0, S_RZ0, S_RZ1, S_RELRES, S_NNZ, S_RESN, S_RHSN, S_COUNT_ };
 static_assert(S_COUNT_ <= SC_COUNT, "scalar slots");
+static_assert(P_COUNT <= 8, "partial slots");
 
+template <class VT, class MT>
 struct CgArgs {
     int nu, cap, maxit, it, npq;   // npq: blocks of the product kernel
     double tol;
     const int32_t* __restrict__ rowlen;
     const int32_t* __restrict__ acol;
-    const double* __restrict__ aval;
+    const MT* __restrict__ aval;
     const double* __restrict__ diag;
-    const double* __restrict__ rhs;
-    double* x;
-    double* r;
-    double* p;
-    double* q;
+    const double* __restrict__ rhs;   // FP64 right-hand side of this solve
+    VT* x;
+    VT* r;
+    VT* p;
+    VT* q;
     double* part;
     double* scal;
     int32_t* flags;
 };
+
+#define PSLOT(a, s) ((a).part + (long)(s) * kMaxPartials)
 
 // Sum of the first n partials of one slot, identical in every block.
 __device__ __forceinline__ double sum_partials(const double* part, int n, double* lds)
@@ -53,18 +65,20 @@ __device__ __forceinline__ double sum_partials(const double* part, int n, double
     return block_sum(s, lds);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs a)
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     double bb = 0.0, rz = 0.0, nz = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
-        double b = a.rhs[r];
-        double z = b / a.diag[r];
-        a.x[r] = 0.0;
-        a.r[r] = b;
+        const double b = a.rhs[r];
+        const VT rv = (VT)b;
+        const VT z = rv / (VT)a.diag[r];
+        a.x[r] = (VT)0;
+        a.r[r] = rv;
         a.p[r] = z;
         bb += b * b;
-        rz += b * z;
+        rz += (double)rv * (double)z;
         nz += (double)(a.rowlen[r] + 1);
     }
     bb = block_sum(bb, lds);
@@ -77,7 +91,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs a)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs a)
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     double bb = sum_partials(PSLOT(a, P_BB), kCgBlocks, lds);
@@ -98,25 +113,27 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs a)
 
 // One thread per row over the sliced layout: a wave's 64 rows read slot k as
 // one contiguous line set; 16 entries' loads are in flight per lane and the
-// row is accumulated in order. The matrix streams from HBM once per product
-// (non-temporal loads keep it from evicting the gathered vector p from L2).
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs a)
+// row is accumulated in order (in VT). The matrix streams from HBM once per
+// product (non-temporal loads keep it from evicting the gathered p from L2).
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     if (a.flags[FLAG_CG_DONE]) return;
     double pq = 0.0;
     const int r = xcd_tid();
     if (r < a.nu) {
-        const double pr = a.p[r];
-        double acc = a.diag[r] * pr;
+        const VT pr = a.p[r];
+        VT acc = (VT)a.diag[r] * pr;
         const int len = a.rowlen[r];
         const int32_t* __restrict__ col = a.acol + sl_index(r, 0, a.cap);
-        const double* __restrict__ val = a.aval + sl_index(r, 0, a.cap);
+        const MT* __restrict__ val = a.aval + sl_index(r, 0, a.cap);
         constexpr int U = 16;
         int k = 0;
         for (; k + U <= len; k += U) {
             int32_t cc[U];
-            double vv[U], pp[U];
+            MT vv[U];
+            VT pp[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 cc[u] = __builtin_nontemporal_load(&col[64L * (k + u)]);
@@ -125,17 +142,18 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs a)
 #pragma unroll
             for (int u = 0; u < U; ++u) pp[u] = a.p[cc[u]];
 #pragma unroll
-            for (int u = 0; u < U; ++u) acc = acc + vv[u] * pp[u];
+            for (int u = 0; u < U; ++u) acc = acc + (VT)vv[u] * pp[u];
         }
-        for (; k < len; ++k) acc = acc + val[64L * k] * a.p[col[64L * k]];
+        for (; k < len; ++k) acc = acc + (VT)val[64L * k] * a.p[col[64L * k]];
         a.q[r] = acc;
-        pq = pr * acc;
+        pq = (double)pr * (double)acc;
     }
     pq = block_sum(pq, lds);
     if (threadIdx.x == 0) PSLOT(a, P_PQ)[blockIdx.x] = pq;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     if (a.flags[FLAG_CG_DONE]) return;
@@ -149,16 +167,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
         }
         return;
     }
-    const double alpha = rz_old / pq;
+    const VT alpha = (VT)(rz_old / pq);
     double rr = 0.0, rz = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
-        const double pr = a.p[r];
+        const VT pr = a.p[r];
         a.x[r] = a.x[r] + alpha * pr;
-        const double res = a.r[r] - alpha * a.q[r];
+        const VT res = a.r[r] - alpha * a.q[r];
         a.r[r] = res;
-        const double z = res / a.diag[r];
-        rr += res * res;
-        rz += res * z;
+        const VT z = res / (VT)a.diag[r];
+        rr += (double)res * (double)res;
+        rz += (double)res * (double)z;
     }
     rr = block_sum(rr, lds);
     rz = block_sum(rz, lds);
@@ -168,7 +186,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs a)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs a)
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     if (a.flags[FLAG_CG_DONE]) return;
@@ -188,10 +207,93 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs a)
         if (lead) a.flags[FLAG_CG_DONE] = 1;
         return;
     }
-    const double beta = rz / a.scal[S_RZ0 + ((a.it - 1) & 1)];
+    const VT beta = (VT)(rz / a.scal[S_RZ0 + ((a.it - 1) & 1)]);
     if (lead) a.scal[S_RZ0 + (a.it & 1)] = rz;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x)
-        a.p[r] = a.r[r] / a.diag[r] + beta * a.p[r];
+        a.p[r] = a.r[r] / (VT)a.diag[r] + beta * a.p[r];
+}
+
+// ---- outer FP64 refinement kernels ---------------------------------------------------------
+struct IrArgs {
+    int nu, cap, npq;
+    const int32_t* __restrict__ rowlen;
+    const int32_t* __restrict__ acol;
+    const double* __restrict__ aval;
+    const double* __restrict__ diag;
+    const double* __restrict__ b;
+    const double* __restrict__ x;
+    double* res;
+    double* part;
+    double* scal;
+};
+
+// res = b - A x in FP64 with the FP64 matrix; partials ||res||^2 and ||b||^2
+// per block.
+__global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
+{
+    __shared__ double lds[kBlock];
+    double rr = 0.0, bb = 0.0;
+    const int r = xcd_tid();
+    if (r < a.nu) {
+        double acc = a.diag[r] * a.x[r];
+        const int len = a.rowlen[r];
+        const int32_t* __restrict__ col = a.acol + sl_index(r, 0, a.cap);
+        const double* __restrict__ val = a.aval + sl_index(r, 0, a.cap);
+        constexpr int U = 16;
+        int k = 0;
+        for (; k + U <= len; k += U) {
+            int32_t cc[U];
+            double vv[U], xx[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                cc[u] = __builtin_nontemporal_load(&col[64L * (k + u)]);
+                vv[u] = __builtin_nontemporal_load(&val[64L * (k + u)]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xx[u] = a.x[cc[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc = acc + vv[u] * xx[u];
+        }
+        for (; k < len; ++k) acc = acc + val[64L * k] * a.x[col[64L * k]];
+        const double b = a.b[r];
+        const double rv = b - acc;
+        a.res[r] = rv;
+        rr = rv * rv;
+        bb = b * b;
+    }
+    rr = block_sum(rr, lds);
+    bb = block_sum(bb, lds);
+    if (threadIdx.x == 0) {
+        PSLOT(a, P_RES)[blockIdx.x] = rr;
+        PSLOT(a, P_RHS)[blockIdx.x] = bb;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ir_residual_fin(IrArgs a)
+{
+    __shared__ double lds[kBlock];
+    const double rr = sum_partials(PSLOT(a, P_RES), a.npq, lds);
+    const double bb = sum_partials(PSLOT(a, P_RHS), a.npq, lds);
+    if (threadIdx.x == 0) {
+        a.scal[S_RESN] = sqrt(rr);
+        a.scal[S_RHSN] = sqrt(bb);
+    }
+}
+
+template <class VT>
+__global__ void k_ir_correct(int nu, double* __restrict__ x, const VT* __restrict__ d)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nu) x[r] = x[r] + (double)d[r];
+}
+
+// Start vector of a warm-started solve: each unknown's pressure of the
+// previous step (the P field), by its particle index.
+__global__ void k_gather_x0(int nu, const int32_t* __restrict__ urow, const double* __restrict__ p,
+                            double* __restrict__ x)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nu) x[r] = p[urow[r]];
 }
 
 __global__ void k_scatter_x(int nu, const int32_t* __restrict__ urow, const double* __restrict__ x,
@@ -201,74 +303,176 @@ __global__ void k_scatter_x(int nu, const int32_t* __restrict__ urow, const doub
     if (r < nu) pnew[urow[r]] = x[r];
 }
 
-int ppe_solve(vox_ctx* c, vox_solve_info* info)
+// ---- host ------------------------------------------------------------------------------------
+// Runs CG on A x = rhs to ||r|| <= tol ||rhs||. Returns iterations; *conv, *brk.
+template <class VT, class MT>
+static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
 {
-    const int nu = c->nu;
-    CgArgs a;
-    a.nu = nu; a.cap = c->a_cap; a.tol = c->p.cg_tol; a.maxit = c->p.cg_maxit; a.it = 0;
-    a.npq = grid_for(nu > 0 ? nu : 1);
-    if (a.npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
-    a.rowlen = c->rowlen; a.acol = c->acol; a.aval = c->aval; a.diag = c->adiag; a.rhs = c->rhs;
-    a.x = c->cg_x; a.r = c->cg_r; a.p = c->cg_p; a.q = c->cg_q;
-    a.part = c->partials; a.scal = c->dscal; a.flags = c->dflags;
-    VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));
     {
         ProfScope ps(c, KC_CG_INIT);
-        k_cg_init<<<kCgBlocks, kBlock, 0, c->stream>>>(a);
-        k_cg_init_fin<<<1, kBlock, 0, c->stream>>>(a);
+        k_cg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+        k_cg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
         VOX_HIP(hipGetLastError());
     }
     // Check schedule: the first batch runs to two iterations short of the last
-    // solve's count (consecutive steps need similar counts), then the flag is
+    // solve's count (consecutive solves need similar counts), then the flag is
     // read after every cg_check iterations (default 1), so no kernel is queued
     // after convergence.
     const int check = c->p.cg_check > 0 ? c->p.cg_check : 1;
-    const int first = c->cg_expect - 2;
+    int& expect = std::is_same<VT, double>::value ? c->cg_expect : c->cg_expect_inner;
+    const int first = expect - 2;
     int it = 0;
     VOX_TRY(read_flags(c));
-    while (!c->hflags[FLAG_CG_DONE] && it < c->p.cg_maxit) {
+    c->nnz_total = (long)c->hscal[S_NNZ];
+    while (!c->hflags[FLAG_CG_DONE] && it < a.maxit) {
         const int batch = (it == 0 && first > check) ? first : check;
-        for (int k = 0; k < batch && it < c->p.cg_maxit; ++k) {
+        for (int k = 0; k < batch && it < a.maxit; ++k) {
             a.it = ++it;
             {
                 ProfScope ps(c, KC_SPMV);
-                k_cg_spmv<<<a.npq, kBlock, 0, c->stream>>>(a);
+                k_cg_spmv<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
             }
             {
                 ProfScope ps(c, KC_CG_UPDATE);
-                k_cg_update<<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_cg_update<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
             }
             {
                 ProfScope ps(c, KC_CG_P);
-                k_cg_p<<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_cg_p<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
             }
         }
         VOX_HIP(hipGetLastError());
         VOX_TRY(read_flags(c));
     }
+    expect = c->hflags[FLAG_CG_ITERS];
+    *iters_out = c->hflags[FLAG_CG_ITERS];
+    return VOX_OK;
+}
+
+template <class VT, class MT>
+static void fill_args(vox_ctx* c, CgArgs<VT, MT>& a, const MT* aval, VT* x, VT* r, VT* p, VT* q,
+                      const double* rhs, double tol, int maxit)
+{
+    a.nu = c->nu; a.cap = c->a_cap; a.tol = tol; a.maxit = maxit; a.it = 0;
+    a.npq = grid_for(c->nu > 0 ? c->nu : 1);
+    a.rowlen = c->rowlen; a.acol = c->acol; a.aval = aval; a.diag = c->adiag; a.rhs = rhs;
+    a.x = x; a.r = r; a.p = p; a.q = q;
+    a.part = c->partials; a.scal = c->dscal; a.flags = c->dflags;
+}
+
+static int finish(vox_ctx* c, vox_solve_info* info, int iters, int conv, double relres,
+                  double bnorm, int brk)
+{
+    const int nu = c->nu;
     if (nu > 0) {
         k_scatter_x<<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->urow, c->cg_x, c->f[VOX_F_PNEW]);
         VOX_HIP(hipGetLastError());
     }
-    const int iters = c->hflags[FLAG_CG_ITERS];
-    const int conv = c->hflags[FLAG_CG_CONV];
-    c->cg_expect = iters;
     c->st.last_iters = iters;
-    c->st.last_relres = c->hscal[S_RELRES];
+    c->st.last_relres = relres;
     if (info) {
         info->iterations = iters;
         info->converged = conv;
         info->unknowns = nu;
-        info->nnz = (int32_t)c->hscal[S_NNZ];
-        info->relres = c->hscal[S_RELRES];
-        info->bnorm = c->hscal[S_BNORM];
+        info->nnz = (int32_t)c->nnz_total;
+        info->relres = relres;
+        info->bnorm = bnorm;
     }
-    if (c->hflags[FLAG_CG_BREAK]) return fail(VOX_ERR_SOLVER, "CG breakdown (p.Ap <= 0 or not finite)");
+    if (brk) return fail(VOX_ERR_SOLVER, "CG breakdown (p.Ap <= 0 or not finite)");
     if (!conv)
-        return fail(VOX_ERR_SOLVER, "CG did not reach the relative residual " +
+        return fail(VOX_ERR_SOLVER, "PPE solve did not reach the relative residual " +
                                         std::to_string(c->p.cg_tol) + " in " +
                                         std::to_string(iters) + " iterations");
     return VOX_OK;
+}
+
+static int solve_fp64(vox_ctx* c, vox_solve_info* info)
+{
+    CgArgs<double, double> a;
+    fill_args(c, a, (const double*)c->aval, c->cg_x, c->cg_r, c->cg_p, c->cg_q, c->rhs,
+              c->p.cg_tol, c->p.cg_maxit);
+    if (a.npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
+    int iters = 0;
+    VOX_TRY(run_cg(c, a, &iters));
+    return finish(c, info, iters, c->hflags[FLAG_CG_CONV], c->hscal[S_RELRES], c->hscal[S_BNORM],
+                  c->hflags[FLAG_CG_BREAK]);
+}
+
+// Refinement driver: x = x0 (zero, or the previous pressure when warm);
+// repeat r = b - A x in FP64 on the FP64 matrix, stop when ||r|| <= tol ||b||,
+// else solve A d = r with CG<VT, MT> to a relative tolerance and x += d.
+// Mixed precision: VT = MT = float, each correction to max(eta, the remaining
+// factor). FP64 warm start: VT = MT = double, one correction normally reaches
+// the target.
+template <class VT, class MT>
+static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x, VT* r, VT* p,
+                         VT* q, double eta, bool warm)
+{
+    const int nu = c->nu;
+    IrArgs ir;
+    ir.nu = nu; ir.cap = c->a_cap; ir.npq = grid_for(nu > 0 ? nu : 1);
+    if (ir.npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
+    ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval; ir.diag = c->adiag;
+    ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
+    if (warm && nu > 0) {
+        k_gather_x0<<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->urow, c->f[VOX_F_P], c->cg_x);
+        VOX_HIP(hipGetLastError());
+    } else {
+        VOX_TRY(fill_f64(c, c->cg_x, 0.0, nu));
+    }
+    const double tol = c->p.cg_tol;
+    int total = 0, conv = 0, brk = 0, outer = 0;
+    double bnorm = 0.0, rel = 1.0;
+    for (;; ++outer) {
+        // From a zero start the first residual is b itself: no product is
+        // needed, and the first inner solve takes b directly.
+        const bool from_b = (outer == 0 && !warm);
+        if (!from_b) {
+            {
+                ProfScope ps(c, KC_IR);
+                k_ir_residual<<<ir.npq, kBlock, 0, c->stream>>>(ir);
+                k_ir_residual_fin<<<1, kBlock, 0, c->stream>>>(ir);
+                VOX_HIP(hipGetLastError());
+            }
+            VOX_TRY(read_flags(c));
+            const double rn = c->hscal[S_RESN];
+            bnorm = c->hscal[S_RHSN];
+            rel = (bnorm > 0) ? rn / bnorm : 0.0;
+            if (rel <= tol) { conv = 1; break; }
+            if (outer >= c->p.ir_maxouter || total >= c->p.cg_maxit || !std::isfinite(rel)) break;
+        }
+        // relative target of this correction: no tighter than the remaining
+        // factor to the goal (half of it, for the rounding of the update)
+        const double need = 0.5 * tol / rel;
+        CgArgs<VT, MT> a;
+        fill_args(c, a, aval, x, r, p, q, from_b ? c->rhs : c->cg_r, eta > need ? eta : need,
+                  c->p.cg_maxit - total);
+        int it = 0;
+        VOX_TRY(run_cg(c, a, &it));
+        total += it;
+        if (c->hflags[FLAG_CG_BREAK] && it == 0) { brk = 1; break; }
+        k_ir_correct<VT><<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->cg_x, x);
+        VOX_HIP(hipGetLastError());
+    }
+    c->ir_outer_last = outer;
+    return finish(c, info, total, conv, rel, bnorm, brk);
+}
+
+static int solve_mixed(vox_ctx* c, vox_solve_info* info)
+{
+    const double eta = c->p.ir_eta > 0 ? c->p.ir_eta : 1e-4;
+    return solve_refined<float, float>(c, info, (const float*)c->aval32, c->cg_x32, c->cg_r32,
+                                       c->cg_p32, c->cg_q32, eta, c->p.warm_start != 0);
+}
+
+int ppe_solve(vox_ctx* c, vox_solve_info* info)
+{
+    VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));
+    if (c->p.precision == VOX_PREC_MIXED) return solve_mixed(c, info);
+    if (c->p.warm_start)
+        return solve_refined<double, double>(c, info, (const double*)c->aval, c->cg_d, c->cg_e,
+                                             c->cg_p, c->cg_q, 0.0, true);
+    return solve_fp64(c, info);
 }
 
 }  // namespace vox

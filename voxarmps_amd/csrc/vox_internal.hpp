@@ -69,17 +69,11 @@ enum DevFlag {
     FLAG_CG_ITERS,
     FLAG_CG_CONV,
     FLAG_CG_BREAK,
-    FLAG_COUNTER0,        // last-block tickets for reductions (several)
-    FLAG_COUNTER1,
-    FLAG_COUNTER2,
     FLAG_COUNT
 };
 
-// device scalar slots of ctx->dscal (double)
-enum DevScal {
-    SC_BB = 0, SC_BNORM, SC_RZ, SC_PQ, SC_ALPHA, SC_BETA, SC_RR, SC_RELRES, SC_NNZ,
-    SC_COUNT
-};
+// device scalar slots of ctx->dscal (double); the solver names its own
+constexpr int SC_COUNT = 12;
 
 // ---- neighbour grid + list ------------------------------------------------------
 struct Grid {
@@ -159,12 +153,23 @@ struct vox_ctx {
     double* rhs = nullptr;         // tp
     int a_cap = 0;
     int nu = 0;
-    int cg_expect = 0;             // iterations of the previous solve
+    long nnz_total = 0;            // stored entries incl. diagonal (last solve)
+    int cg_expect = 0;             // iterations of the previous FP64 solve
+    int cg_expect_inner = 0;       // iterations of the previous FP32 inner solve
+    int ir_outer_last = 0;
     // CG vectors over unknowns
     double* cg_x = nullptr;
     double* cg_r = nullptr;
     double* cg_p = nullptr;
     double* cg_q = nullptr;
+    double* cg_d = nullptr;        // FP64 inner solve of a warm-started solve: x
+    double* cg_e = nullptr;        //   and residual
+    // mixed precision: FP32 matrix values and inner-solve vectors
+    float* aval32 = nullptr;       // same sliced layout as aval
+    float* cg_x32 = nullptr;
+    float* cg_r32 = nullptr;
+    float* cg_p32 = nullptr;
+    float* cg_q32 = nullptr;
 
     // collision exact-emulation buffers
     int32_t* coll_cnt = nullptr;   // tp
@@ -185,7 +190,7 @@ struct vox_ctx {
     int32_t* blockmax = nullptr;   // one slot per 256-particle block
 
     // reductions
-    double* partials = nullptr;    // 4 * kMaxPartials
+    double* partials = nullptr;    // 8 * kMaxPartials
     double* dscal = nullptr;       // SC_COUNT
     int32_t* dflags = nullptr;     // FLAG_COUNT
     int32_t* hflags = nullptr;     // pinned mirror
@@ -207,7 +212,7 @@ namespace vox {
 enum KClass {
     KC_CELL = 0, KC_SORT, KC_LIST, KC_PND, KC_SPS, KC_VFRAC, KC_VISC, KC_PRED,
     KC_BCON, KC_ASSEMBLE, KC_CG_INIT, KC_SPMV, KC_CG_UPDATE, KC_CG_P, KC_PHAT,
-    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_COUNT
+    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_IR, KC_COUNT
 };
 const char* kclass_name(int k);
 int kclass_from_name(const char* name);
@@ -248,9 +253,11 @@ int st_sps(vox_ctx* c);
 int st_vfraction(vox_ctx* c);
 int st_viscosity(vox_ctx* c);
 int st_prediction(vox_ctx* c);
+int st_prediction32(vox_ctx* c);
 int st_bcon(vox_ctx* c);
 int st_phat(vox_ctx* c);
 int st_pressgrad(vox_ctx* c);
+int st_pressgrad32(vox_ctx* c);
 int st_euler(vox_ctx* c);
 int st_collision(vox_ctx* c);
 int st_prepdata(vox_ctx* c);

@@ -112,6 +112,28 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     a.rhs[r] = src;
 }
 
+// FP32 copy of the off-diagonal values for the mixed-precision inner solves:
+// one thread per row, so a wave's 64 rows read and write slot k of their
+// slice as one contiguous line set.
+__global__ void k_aval32(int nu, int cap, const int32_t* __restrict__ rowlen,
+                         const double* __restrict__ aval, float* __restrict__ aval32)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nu) return;
+    const long ob = sl_index(r, 0, cap);
+    const int len = rowlen[r];
+    constexpr int U = 8;
+    int k = 0;
+    for (; k + U <= len; k += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(&aval[ob + 64L * (k + u)]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) aval32[ob + 64L * (k + u)] = (float)v[u];
+    }
+    for (; k < len; ++k) aval32[ob + 64L * k] = (float)aval[ob + 64L * k];
+}
+
 int ppe_assemble(vox_ctx* c)
 {
     ProfScope ps(c, KC_ASSEMBLE);
@@ -124,8 +146,13 @@ int ppe_assemble(vox_ctx* c)
         c->aval = nullptr;
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(n, L.cap)));
+        if (c->aval32) (void)hipFree(c->aval32);
+        c->aval32 = nullptr;
         c->a_cap = L.cap;
     }
+    const bool mixed = (c->p.precision == VOX_PREC_MIXED);
+    if (mixed && !c->aval32)
+        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(n, c->a_cap)));
     k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
     VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
     k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
@@ -149,6 +176,11 @@ int ppe_assemble(vox_ctx* c)
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
+    if (mixed && c->nu > 0) {
+        k_aval32<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen, c->aval,
+                                                           c->aval32);
+        VOX_HIP(hipGetLastError());
+    }
     return VOX_OK;
 }
 

@@ -440,6 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
 
 int st_prediction(vox_ctx* c)
 {
+    if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
     ProfScope ps(c, KC_PRED);
     const vox_params& p = c->p;
     PredArgs a;
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
 
 int st_pressgrad(vox_ctx* c)
 {
+    if (c->p.precision == VOX_PREC_MIXED) return st_pressgrad32(c);
     ProfScope ps(c, KC_PGRAD);
     const vox_params& p = c->p;
     GradArgs a;

@@ -1,0 +1,289 @@
+// vox_stages32.hip — FP32 twins of the two velocity stages for the mixed
+// precision mode: the predictor (viscous + explicit pressure terms) and the
+// pressure-gradient correction.
+//
+// The particle state stays FP64: positions are integrated in FP64 and the
+// particle number densities, which feed the PPE source (n* - n0)/n0 through a
+// cancellation, keep their FP64 kernels. Each FP32 stage packs the fields its
+// neighbour loop gathers into one record of fixed-point positions and FP32
+// velocities and pressures. A position is stored as an unsigned 32-bit
+// fraction of the grid box along each axis: the difference of two, taken
+// modulo 2^32 as a signed integer, is exact (neighbours are far closer than
+// half the box), so a neighbour offset keeps 2^-32 of the box extent before
+// its single rounding to FP32. (A difference of FP32 positions would lose
+// |x|/|dx|, 100-300 ulps in a metre-sized tank.) Weights, distances and the
+// neighbour sums are FP32; outputs go back to the FP64 fields. Formulas and
+// their order follow the FP64 kernels (PREDICTION functions.cu:2283-2369,
+// PRESSGRAD 245-355).
+#include "vox_internal.hpp"
+
+namespace vox {
+
+struct __align__(16) Pred32 { uint32_t x, y, z; float u, v, w, pn, meu; };   // 32 B
+struct __align__(8) Grad32 { uint32_t x, y, z; float pn, ph, pad; };          // 24 B
+static_assert(sizeof(Pred32) == 32 && sizeof(Grad32) == 24, "record sizes");
+
+// Fixed-point frame: origin and 2^32/extent per axis (packing), extent/2^32
+// per axis (differences).
+struct Quant {
+    double ox, oy, oz, ix, iy, iz;
+    float sx, sy, sz;
+};
+
+static Quant make_quant(const vox_params& p)
+{
+    Quant q;
+    const double two32 = 4294967296.0;
+    const double ex = p.xmax - p.xmin, ey = p.ymax - p.ymin, ez = p.zmax - p.zmin;
+    q.ox = p.xmin; q.oy = p.ymin; q.oz = p.zmin;
+    q.ix = ex > 0 ? two32 / ex : 0.0;
+    q.iy = ey > 0 ? two32 / ey : 0.0;
+    q.iz = (p.dim == 3 && ez > 0) ? two32 / ez : 0.0;
+    q.sx = (float)(ex > 0 ? ex / two32 : 0.0);
+    q.sy = (float)(ey > 0 ? ey / two32 : 0.0);
+    q.sz = (float)((p.dim == 3 && ez > 0) ? ez / two32 : 0.0);
+    return q;
+}
+
+__device__ __forceinline__ uint32_t qpos(double v, double o, double inv)
+{
+    double t = (v - o) * inv;
+    t = t < 0.0 ? 0.0 : (t > 4294967295.0 ? 4294967295.0 : t);
+    return (uint32_t)rint(t);
+}
+
+__device__ __forceinline__ float qdiff(uint32_t a, uint32_t b, float s)
+{
+    return (float)(int32_t)(a - b) * s;
+}
+
+__device__ __forceinline__ float wfun32(float r, int ktype, float re)
+{
+    float q = r / re;
+    if (ktype == 2) return (q < 1.0f) ? (1.0f / q) - 1.0f : 0.0f;
+    if (ktype == 6) return (q <= 1.0f) ? (1.0f - q) * (1.0f - q) * (1.0f - q) : 0.0f;
+    if (ktype == 1) {
+        if (q < 0.5f) return 2.0f - 4.0f * (q * q);
+        if (q <= 1.0f) return (2.0f * q - 2.0f) * (2.0f * q - 2.0f);
+        return 0.0f;
+    }
+    if (ktype == 5) return (q <= 1.0f) ? 1.5f * logf(1.0f / (q + 0.000001f)) : 0.0f;
+    return 0.0f;
+}
+
+// ---- PREDICTION ------------------------------------------------------------------------------
+struct Pred32Args {
+    int n, dim, ktype;
+    double dt64;
+    Quant q;
+    float re_s, re_l, rho1, rho2, gx, gy, gz, dt, lapc, ln0, gcoef_num, n0s, relax1;
+    ListView L;
+    const int32_t* pt;
+    const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
+    double *xs, *ys, *zs, *us, *vs, *ws;
+    Pred32* rec;
+};
+
+__global__ void k_pack_pred32(Pred32Args a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    Pred32 r;
+    r.x = qpos(a.x[i], a.q.ox, a.q.ix);
+    r.y = qpos(a.y[i], a.q.oy, a.q.iy);
+    r.z = (a.dim == 3) ? qpos(a.z[i], a.q.oz, a.q.iz) : 0u;
+    r.u = (float)a.u[i]; r.v = (float)a.v[i]; r.w = (a.dim == 3) ? (float)a.w[i] : 0.0f;
+    r.pn = (float)a.pn[i]; r.meu = (float)a.meu[i];
+    a.rec[i] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
+{
+    const int i = xcd_tid();
+    if (i >= a.n) return;
+    const int dim = a.dim;
+    const int ti = a.pt[i];
+    if (ti <= 0) {
+        a.xs[i] = a.x[i]; a.ys[i] = a.y[i]; a.us[i] = a.u[i]; a.vs[i] = a.v[i];
+        if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
+        return;
+    }
+    const Pred32 me = a.rec[i];
+    const float mi = me.meu, phi_ = (float)a.ph[i];
+    const float rhoN = (ti == 1) ? a.rho1 : a.rho2;
+    float s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
+    const int cnt = a.L.cnt[i];
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    constexpr int U = 8;
+    for (int l0 = 0; l0 < cnt; l0 += U) {
+        int jj[U], tj[U];
+        Pred32 rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const Pred32 rj = rv[u];
+            const float dx = qdiff(rj.x, me.x, a.q.sx), dy = qdiff(rj.y, me.y, a.q.sy),
+                        dz = qdiff(rj.z, me.z, a.q.sz);
+            const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+            const bool use = (jj[u] != i) && (D > 0.0f);
+            const float hm = 2.0f * mi * rj.meu / (mi + rj.meu) / rhoN;
+            const float NEU = (ti == tj[u]) ? mi / rhoN : hm;
+            const float wg = wfun32(D, a.ktype, a.re_s);
+            const float wl = (a.re_l == a.re_s) ? wg : wfun32(D, a.ktype, a.re_l);
+            const float dp = rj.pn - phi_;
+            const float du = rj.u - me.u, dv = rj.v - me.v, dw = rj.w - me.w;
+            const float g = wg / D / D;
+            s1 = use ? s1 + dp * dx * g : s1;
+            s2 = use ? s2 + dp * dy * g : s2;
+            s3 = use ? s3 + dp * dz * g : s3;
+            s4 = use ? s4 + wl * du * NEU : s4;
+            s5 = use ? s5 + wl * dv * NEU : s5;
+            s6 = use ? s6 + wl * dw * NEU : s6;
+            s7 = use ? s7 + wl * du : s7;
+            s8 = use ? s8 + wl * dv : s8;
+            s9 = use ? s9 + wl * dw : s9;
+        }
+    }
+    const float nt = (float)a.neut[i];
+    const float gc = a.relax1 * (a.gcoef_num * a.dt / a.n0s / (float)a.rho[i]);
+    const float dt = a.dt;
+    const float us = me.u + a.gx * dt + a.lapc * dt * (s4 + nt * s7) / a.ln0 + dt * (float)a.t1[i] - gc * s1;
+    const float vs = me.v + a.gy * dt + a.lapc * dt * (s5 + nt * s8) / a.ln0 + dt * (float)a.t2[i] - gc * s2;
+    a.us[i] = (double)us;
+    a.vs[i] = (double)vs;
+    a.xs[i] = a.x[i] + a.dt64 * (double)us;
+    a.ys[i] = a.y[i] + a.dt64 * (double)vs;
+    if (dim == 3) {
+        const float ws = me.w + a.gz * dt + a.lapc * dt * (s6 + nt * s9) / a.ln0 + dt * (float)a.t3[i] - gc * s3;
+        a.ws[i] = (double)ws;
+        a.zs[i] = a.z[i] + a.dt64 * (double)ws;
+    }
+}
+
+int st_prediction32(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PRED);
+    const vox_params& p = c->p;
+    Pred32Args a;
+    a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype;
+    a.dt64 = p.dt;
+    a.q = make_quant(p);
+    a.re_s = (float)p.re_s; a.re_l = (float)p.re_l;
+    a.rho1 = (float)p.rho1; a.rho2 = (float)p.rho2;
+    a.gx = (float)p.gx; a.gy = (float)p.gy; a.gz = (float)p.gz; a.dt = (float)p.dt;
+    a.lapc = p.coef2d ? 4.0f : 2.0f * p.dim;
+    a.ln0 = (float)(p.lambda * c->n0l);
+    a.gcoef_num = p.coef2d ? 2.0f : (float)p.dim;
+    a.n0s = (float)c->n0s;
+    a.relax1 = (float)(1 - p.relaxp);
+    a.L = view(c, VOX_LIST_L); a.pt = c->ptype;
+    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
+    a.u = c->f[VOX_F_U]; a.v = c->f[VOX_F_V]; a.w = c->f[VOX_F_W];
+    a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.meu = c->f[VOX_F_MEU];
+    a.neut = c->f[VOX_F_NEUT]; a.rho = c->f[VOX_F_RHO];
+    a.t1 = c->f[VOX_F_TURB1]; a.t2 = c->f[VOX_F_TURB2]; a.t3 = c->f[VOX_F_TURB3];
+    a.xs = c->f[VOX_F_XSTAR]; a.ys = c->f[VOX_F_YSTAR]; a.zs = c->f[VOX_F_ZSTAR];
+    a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
+    a.rec = reinterpret_cast<Pred32*>(c->pred_rec);          // 64-byte slots hold 32-byte records
+    k_pack_pred32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    k_prediction32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- PRESSGRAD ------------------------------------------------------------------------------
+struct Grad32Args {
+    int first, n, dim, ktype, kh;
+    Quant q;
+    float re, gnum, dt, n0, relaxp, vmax;
+    ListView L;
+    const double *x, *y, *z, *pn, *ph, *rho, *us, *vs, *ws;
+    double *un, *vn, *wn;
+    Grad32* rec;
+};
+
+__global__ void k_pack_grad32(Grad32Args a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    Grad32 r;
+    r.x = qpos(a.x[i], a.q.ox, a.q.ix);
+    r.y = qpos(a.y[i], a.q.oy, a.q.iy);
+    r.z = (a.dim == 3) ? qpos(a.z[i], a.q.oz, a.q.iz) : 0u;
+    r.pn = (float)a.pn[i]; r.ph = (float)a.ph[i]; r.pad = 0.0f;
+    a.rec[i] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_pressgrad32(Grad32Args a)
+{
+    const int i = a.first + xcd_tid();
+    if (i >= a.n) return;
+    const Grad32 me = a.rec[i];
+    float s1 = 0, s2 = 0, s3 = 0;
+    const int cnt = a.L.cnt[i];
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    constexpr int U = 8;
+    for (int l0 = 0; l0 < cnt; l0 += U) {
+        int jj[U];
+        Grad32 rv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+#pragma unroll
+        for (int u = 0; u < U; ++u) rv[u] = a.rec[jj[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const Grad32 rj = rv[u];
+            const float dx = qdiff(rj.x, me.x, a.q.sx), dy = qdiff(rj.y, me.y, a.q.sy),
+                        dz = qdiff(rj.z, me.z, a.q.sz);
+            const float D = sqrtf(dx * dx + dy * dy + dz * dz);
+            const float wv = wfun32(D, a.ktype, a.re);
+            const float dp = (a.kh == 1) ? (rj.pn + me.pn - (me.ph + rj.ph)) : (rj.pn - me.ph);
+            const float g = wv / D / D;
+            const bool use = jj[u] != i;
+            s1 = use ? s1 + dp * dx * g : s1;
+            s2 = use ? s2 + dp * dy * g : s2;
+            s3 = use ? s3 + dp * dz * g : s3;
+        }
+    }
+    const float g = a.relaxp * (a.gnum * a.dt / a.n0 / (float)a.rho[i]);
+    float un = (float)a.us[i] - g * s1;
+    float vn = (float)a.vs[i] - g * s2;
+    const float cf = 2.0f;
+    if (fabsf(un) > cf * a.vmax) un = a.vmax;                  // damper 345-351
+    if (vn > cf * a.vmax) vn = cf * a.vmax;
+    if (vn < -cf * a.vmax) vn = -cf * a.vmax;
+    a.un[i] = (double)un;
+    a.vn[i] = (double)vn;
+    if (a.dim == 3) {
+        float wn = (float)a.ws[i] - g * s3;
+        if (fabsf(wn) > cf * a.vmax) wn = a.vmax;
+        a.wn[i] = (double)wn;
+    }
+}
+
+int st_pressgrad32(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PGRAD);
+    const vox_params& p = c->p;
+    Grad32Args a;
+    a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype;
+    a.q = make_quant(p);
+    a.kh = p.kh_correction; a.re = (float)p.re_s; a.gnum = p.coef2d ? 2.0f : (float)p.dim;
+    a.dt = (float)p.dt; a.n0 = (float)c->n0s; a.relaxp = (float)p.relaxp; a.vmax = (float)p.vmax;
+    a.L = view(c, VOX_LIST_S);
+    a.x = c->f[VOX_F_XSTAR]; a.y = c->f[VOX_F_YSTAR]; a.z = c->f[VOX_F_ZSTAR];
+    a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.rho = c->f[VOX_F_RHO];
+    a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
+    a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW];
+    a.rec = reinterpret_cast<Grad32*>(c->pred_rec);          // 64-byte slots hold 24-byte records
+    k_pack_grad32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    const int m = c->tp - a.first;
+    if (m > 0) k_pressgrad32<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+}  // namespace vox

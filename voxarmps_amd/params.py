@@ -47,6 +47,9 @@ def default_params(dim, dl):
     p.cg_check = 1
     p.cap_s = 0
     p.cap_l = 0
+    p.ir_eta = 1e-4
+    p.ir_maxouter = 8
+    p.warm_start = 0
     return p
 
 
