@@ -117,6 +117,9 @@ typedef struct vox_params {
     int32_t warm_start;   /* 1: the PPE solve starts from each unknown's pressure
                              of the previous step (field P) and refines in FP64;
                              0: from zero, as the reference (functions.cu:2503) */
+    int32_t list_mode;    /* neighbour search: 0 auto, 1 one thread per particle,
+                             2 one wave per grid cell (same lists either way)    */
+    int32_t reserved1;
 } vox_params;
 
 /* PPE solve report */

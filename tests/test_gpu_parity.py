@@ -71,8 +71,13 @@ def _cases():
     yield "3d-ref-radius", r3, p3
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2], ids=["auto", "per-particle", "per-cell"])
 @pytest.mark.parametrize("name,case,p", list(_cases()), ids=lambda v: v if isinstance(v, str) else "")
-def test_neighbor_lists_bit_exact(hip, oracle, name, case, p):
+def test_neighbor_lists_bit_exact(hip, oracle, name, case, p, mode):
+    """Both search kernels (one thread per particle, one wave per cell) give
+    the oracle's lists."""
+    p = p.copy()
+    p.list_mode = mode
     g, c = _pair(case, p, hip, oracle)
     g.NEIGHBOR()
     c.NEIGHBOR()
@@ -269,11 +274,19 @@ def test_gpu_runs_are_deterministic(hip):
         assert np.array_equal(a, b)
 
 
-def test_row_capacity_grows(hip, oracle):
-    case = vx.jitter(vx.dam_break_2d_c0(), 0.001, seed=2)
-    p = vx.dam2d_params(case)
+@pytest.mark.parametrize("dim,mode", [(2, 1), (2, 2), (3, 2)])
+def test_row_capacity_grows(hip, oracle, dim, mode):
+    """Rows that overflow a too-small capacity are detected, storage grows and
+    the search repeats, in both search kernels."""
+    if dim == 2:
+        case = vx.jitter(vx.dam_break_2d_c0(), 0.001, seed=2)
+        p = vx.dam2d_params(case)
+    else:
+        case = vx.jitter(vx.dam_break(3, 0.005, (16, 16, 12), (48, 20, 13)), 0.0005, seed=3)
+        p = vx.dam3d_params(case)
     p.cap_s = 4
     p.cap_l = 8
+    p.list_mode = mode
     g, c = _pair(case, p, hip, oracle)
     g.NEIGHBOR()
     c.NEIGHBOR()

@@ -54,6 +54,7 @@ class VoxParams(C.Structure):
         ("cg_tol", _f64),
         ("cg_maxit", _i32), ("cg_check", _i32), ("cap_s", _i32), ("cap_l", _i32),
         ("ir_eta", _f64), ("ir_maxouter", _i32), ("warm_start", _i32),
+        ("list_mode", _i32), ("reserved1", _i32),
     ]
 
     def copy(self):

@@ -208,6 +208,8 @@ int vox_params_default(vox_params* p, int dim, double dl)
     p->ir_eta = 1e-4;
     p->ir_maxouter = 8;
     p->warm_start = 0;
+    p->list_mode = 0;
+    p->reserved1 = 0;
     return VOX_OK;
 }
 
@@ -248,6 +250,7 @@ static int validate(const vox_params* p)
     if (!(p->ir_eta >= 0.0 && p->ir_eta < 1.0) || p->ir_maxouter < 1 ||
         (p->warm_start != 0 && p->warm_start != 1))
         return fail(VOX_ERR_ARG, "ir_eta must be in [0, 1), ir_maxouter >= 1, warm_start 0 or 1");
+    if (p->list_mode < 0 || p->list_mode > 2) return fail(VOX_ERR_ARG, "list_mode must be 0, 1 or 2");
     if (p->ktype != 2 && p->ktype != 6 && p->ktype != 1 && p->ktype != 5)
         return fail(VOX_ERR_ARG, "unsupported kernel type");
     return VOX_OK;

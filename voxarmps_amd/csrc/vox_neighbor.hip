@@ -325,6 +325,83 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     (void)flag_max;
 }
 
+// Cell-tiled stencil walk for large cells (3D list L: 64-80 particles per
+// cell). Every particle of a cell has the same candidate sequence (the stencil
+// cells in reference order, ascending id inside each), so one wave takes up to
+// 64 particles of a cell, streams each chunk of 64 candidate records once into
+// LDS and every lane tests every candidate (broadcast reads): the records are
+// read once per wave instead of once per particle. Block b serves cell b / 2;
+// its half g = b % 2 takes the cell's particles 64 g + 128 k. The accept test,
+// the order and the row storage are those of k_build_list.
+constexpr int kCellHalves = 2;
+
+__global__ __launch_bounds__(kWave) void k_build_list_cells(
+    Grid g, double R, const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
+    int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
+    int flag_overflow, int flag_max)
+{
+    __shared__ PosRec tile[kWave];
+    const int b = xcd_block();
+    const long cellid = b / kCellHalves;
+    const int half = b % kCellHalves;
+    const int lane = threadIdx.x;
+    const int s = start[cellid], m = start[cellid + 1] - s;
+    if (m <= half * kWave) return;                     // uniform over the block
+    const int ncxy = g.ncx * g.ncy;
+    const int ic = (int)(cellid % g.ncx) + 1;
+    const int jc = (int)((cellid / g.ncx) % g.ncy) + 1;
+    const int kc = (int)(cellid / ncxy) + 1;
+    const int m1 = (ic == 1) ? 0 : -1, m2 = (ic == g.ncx) ? 0 : 1;
+    const int m3 = (jc == 1) ? 0 : -1, m4 = (jc == g.ncy) ? 0 : 1;
+    int m5 = 0, m6 = 0;
+    if (g.dim == 3) { m5 = (kc == 1) ? 0 : -1; m6 = (kc == g.ncz) ? 0 : 1; }
+    const double R2 = R * R;
+    const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
+    int kmax = 0;
+    for (int base = half * kWave; base < m; base += kCellHalves * kWave) {
+        const int t = base + lane;
+        const bool active = t < m;
+        const PosRec me = rec[s + (active ? t : 0)];
+        const int i = me.id;
+        const long rb = sl_index(i, 0, cap);
+        int k = 0;
+        for (int row = m1; row <= m2; ++row)
+            for (int col = m3; col <= m4; ++col)
+                for (int dep = m5; dep <= m6; ++dep) {
+                    const int cn = (ic - 1 + row) + (jc - 1 + col) * g.ncx + (kc - 1 + dep) * ncxy;
+                    const int cs = start[cn], ce = start[cn + 1];
+                    for (int j0 = cs; j0 < ce; j0 += kWave) {
+                        const int nc = min(kWave, ce - j0);
+                        __syncthreads();                   // last chunk's reads are done
+                        if (lane < nc) tile[lane] = rec[j0 + lane];
+                        __syncthreads();
+#pragma unroll 4
+                        for (int u = 0; u < nc; ++u) {
+                            const PosRec cu = tile[u];
+                            // z is 0 in 2D records: dx^2+dy^2+0 equals the 2D sum
+                            const double dx = cu.x - me.x, dy = cu.y - me.y, dz = cu.z - me.z;
+                            const double d2 = dx * dx + dy * dy + dz * dz;
+                            bool acc = d2 < lo;
+                            if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
+                                acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                            acc = acc && active && (g.dim == 3 || cu.id != i);
+                            if (acc) {
+                                if (k < cap) ids[rb + 64L * k] = cu.id;
+                                ++k;
+                            }
+                        }
+                    }
+                }
+        if (active) {
+            cnt[i] = k;
+            if (k > cap) atomicAdd(&flags[flag_overflow], 1);
+            kmax = max(kmax, k);
+        }
+    }
+    kmax = wave_max_i32(kmax);
+    if (lane == 0) atomicMax(&flags[flag_max], kmax);
+}
+
 __global__ void k_reduce_max(const int32_t* __restrict__ v, int n, int32_t* __restrict__ out)
 {
     __shared__ int lds[kBlock];
@@ -390,15 +467,28 @@ static int build_one(vox_ctx* c, int which, double re)
         VOX_HIP(hipGetLastError());
     }
     double R = (c->p.dim == 3) ? re + c->p.delta : re;
+    // Walk per cell when cells hold enough particles to fill most of a wave
+    // (expected count = cell volume / DL^dim), else per particle.
+    const double per_cell = (c->p.dim == 3)
+        ? L.g.hx * L.g.hx * L.g.hz / (c->p.dl * c->p.dl * c->p.dl)
+        : L.g.hx * L.g.hx / (c->p.dl * c->p.dl);
+    const bool cell_walk = (c->p.list_mode == 2) ||
+                           (c->p.list_mode == 0 && per_cell >= 24.0 && L.g.tnc * kCellHalves < (1L << 31));
     for (int attempt = 0; attempt < 4; ++attempt) {
         {
             ProfScope ps(c, KC_LIST);
             VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
-            const int nb = grid_for(n);
-            k_build_list<<<nb, kBlock, 0, c->stream>>>(
-                L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
-                L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
-            k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
+            if (cell_walk) {
+                VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
+                k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
+                    L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+            } else {
+                const int nb = grid_for(n);
+                k_build_list<<<nb, kBlock, 0, c->stream>>>(
+                    L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
+                    L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
+                k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
+            }
             VOX_HIP(hipGetLastError());
         }
         VOX_TRY(read_flags(c));

@@ -50,6 +50,7 @@ def default_params(dim, dl):
     p.ir_eta = 1e-4
     p.ir_maxouter = 8
     p.warm_start = 0
+    p.list_mode = 0
     return p
 
 
