@@ -60,6 +60,71 @@ def cpu_baseline(sample_scale=2, steps=2):
     }
 
 
+def calibrate_copy(mib):
+    """Five device-to-device copies of a known size through torch's copy
+    kernel: each reads and writes exactly mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": 5}))
+
+
+def precision_report(checkpoints, scale=1):
+    """The mixed-precision run against the all-FP64 run of the same workload,
+    both on the GPU from the same initial state: relative L2 and Linf
+    differences of the pressure field (and of the fluid positions) at each
+    checkpoint step. Prints one JSON line per checkpoint."""
+    import numpy as np
+
+    import voxarmps_amd as vx
+
+    be = vx.hip_backend()
+    case = vx.dam_break_3d_c1(scale=scale)
+    p64 = vx.dam3d_params(case)
+    p32 = p64.copy()
+    p32.precision = vx._abi.PREC_MIXED
+    a = vx.Simulation(p64, case, backend=be)
+    b = vx.Simulation(p32, case, backend=be)
+    a.init()
+    b.init()
+    done = 0
+    for cp in sorted(checkpoints):
+        while done < cp:
+            n = min(cp - done, 10)
+            t0 = time.perf_counter()
+            a.step(n)
+            t1 = time.perf_counter()
+            b.step(n)
+            t2 = time.perf_counter()
+            done += n
+            sa, sb = a.status(), b.status()
+            print(f"# step {done}: fp64 {1e3 * (t1 - t0) / n:.1f} ms/step iters {sa.last_iters} "
+                  f"coll {sa.collisions}/{sa.collision_cand} | mixed {1e3 * (t2 - t1) / n:.1f} ms/step iters "
+                  f"{sb.last_iters} coll {sb.collisions}/{sb.collision_cand}", file=sys.stderr, flush=True)
+        pa, pb = a.get("P"), b.get("P")
+        d = pb - pa
+        f = case.ptype > 0
+        dx = max(float(np.max(np.abs(b.get(k)[f] - a.get(k)[f]))) for k in ("X", "Y", "Z"))
+        print(json.dumps({
+            "step": cp, "particles": case.tp,
+            "p_l2_rel": float(np.linalg.norm(d) / max(np.linalg.norm(pa), 1e-300)),
+            "p_linf_rel": float(np.max(np.abs(d)) / max(np.max(np.abs(pa)), 1e-300)),
+            "p_linf_abs_pa": float(np.max(np.abs(d))),
+            "p_max_pa": float(np.max(np.abs(pa))),
+            "fluid_pos_linf_m": dx,
+            "iters_fp64": a.status().last_iters, "iters_mixed": b.status().last_iters,
+            "relres_mixed": b.status().last_relres,
+        }), flush=True)
+    a.close()
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,10 +137,23 @@ def main():
                          "velocity stages and FP32 inner CG inside FP64 iterative refinement")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
+    ap.add_argument("--precision-report", type=str, default="", metavar="STEPS",
+                    help="instead of the benchmark, run the FP64 and the mixed-precision "
+                         "simulations side by side and print the pressure-field L2 and Linf "
+                         "differences at the listed steps (comma separated, e.g. 100,500,2000)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, run 5 plain device copies of MIB MiB "
+                         "(reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+    if args.precision_report:
+        precision_report([int(s) for s in args.precision_report.split(",")], args.scale)
+        return
 
     import voxarmps_amd as vx
 
@@ -122,7 +200,8 @@ def main():
     bytes_launch = spmv_bytes(nu, nnz, 4 if mixed else 8)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     per_kernel = {}
-    for k in ("list", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p", "ir_residual",
+    for k in ("list", "list_l", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p",
+              "ir_residual",
               "collision", "pgrad", "phat", "prep", "euler", "bcon", "visc", "vfrac", "misc"):
         t = C.c_double(0.0)
         n = C.c_int64(0)

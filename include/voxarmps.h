@@ -248,7 +248,10 @@ int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
 
 /* ---- profiling ----------------------------------------------------------- */
 /* When enabled, HIP events bracket every launch of the named kernel classes on
- * the context's stream; vox_prof_get returns total ms and launch count. */
+ * the context's stream; vox_prof_get returns total ms and launch count. Class
+ * names: cell, sort, list (list S search), list_l (list L search), pnd, sps,
+ * vfrac, visc, pred, bcon, assemble, cg_init, spmv, cg_update, cg_p,
+ * ir_residual, phat, pgrad, euler, collision, prep, misc. */
 int vox_prof_enable(vox_ctx* ctx, int on);
 int vox_prof_get(vox_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int vox_prof_reset(vox_ctx* ctx);

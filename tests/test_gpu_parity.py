@@ -227,6 +227,46 @@ def test_collision_replay_bit_exact(hip, oracle, dim, vscale):
     c.close()
 
 
+def _sparse_pairs_case(seed, dim, side):
+    """A lattice at l0 with U(-0.36 l0, 0.36 l0) jitter: scattered close pairs,
+    so the collision candidates form many small components."""
+    rng = np.random.default_rng(seed)
+    l0 = 0.01
+    g = np.stack(np.meshgrid(*[np.arange(side)] * dim, indexing="ij"), -1).reshape(-1, dim)
+    pos = 0.1 + g * l0 + rng.uniform(-0.36, 0.36, g.shape) * l0
+    n = pos.shape[0]
+    x, y = pos[:, 0], pos[:, 1]
+    z = pos[:, 2] if dim == 3 else np.zeros(n)
+    zero = np.zeros(n)
+    case = vx.Case(dim, l0, x, y, z, zero.copy(), zero.copy(), zero.copy(), zero.copy(),
+                   np.ones(n, np.int32))
+    p = vx.set_box(vx.default_params(dim, l0), case)
+    return case, p, rng
+
+
+@pytest.mark.parametrize("dim,side,vscale", [(2, 60, 2.0), (3, 16, 2.0), (3, 16, 60.0)])
+def test_collision_components_bit_exact(hip, oracle, dim, side, vscale):
+    """Scattered close pairs: the replay runs one thread per candidate
+    component and must equal the sequential reference loop bit for bit; the
+    fast case (vscale 60) also forces the widened-screen retry."""
+    case, p, rng = _sparse_pairs_case(21 + dim, dim, side)
+    p.dt = 1e-4
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    vel = {n: rng.normal(0, vscale, case.tp) for n in ("UNEW", "VNEW", "WNEW")}
+    for sim in (g, c):
+        for n, v in vel.items():
+            sim.set(n, v)
+        sim.COLLISION2()
+    _assert_same(g, c, FLOAT_OUT["COLLISION2"], f"collision components d{dim}")
+    sg, sc = g.status(), c.status()
+    assert sg.collisions == sc.collisions > 20
+    assert sg.collision_cand > 0
+    g.close()
+    c.close()
+
+
 @pytest.mark.parametrize("nsteps", [40])
 def test_trajectory_2d(hip, oracle, nsteps):
     case = vx.jitter(vx.dam_break_2d_c0(), 0.05 * 0.01, seed=4)

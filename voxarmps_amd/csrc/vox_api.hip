@@ -23,7 +23,7 @@ int fail(int code, const std::string& msg)
 static const char* kKNames[KC_COUNT] = {
     "cell", "sort", "list", "pnd", "sps", "vfrac", "visc", "pred", "bcon", "assemble",
     "cg_init", "spmv", "cg_update", "cg_p", "phat", "pgrad", "euler", "collision", "prep", "misc",
-    "ir_residual"};
+    "ir_residual", "list_l"};
 
 const char* kclass_name(int k) { return (k >= 0 && k < KC_COUNT) ? kKNames[k] : "?"; }
 int kclass_from_name(const char* name)
@@ -306,7 +306,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         (rc = dalloc(&c->cg_e, n)))
         return bail(rc);
     if ((rc = dalloc(&c->coll_cnt, n)) || (rc = dalloc(&c->coll_off, n + 1)) ||
-        (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
+        (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_root, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
         (rc = dalloc(&c->coll_logn, 1)))
         return bail(rc);
     if ((rc = dalloc(&c->partials, 8 * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
@@ -340,7 +340,7 @@ int vox_destroy(vox_ctx* c)
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
     dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
-    dfree(c->coll_stamp); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
+    dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
     dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
     if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->hscal) (void)hipHostFree(c->hscal);

@@ -21,7 +21,7 @@
 // its norm in FP64 on the FP64 matrix, and each correction solves A d = r with
 // FP32 matrix values and FP32 vectors (FP64 dots) to a relative eta.
 //
-// The matrix is sliced ELL (SELL-64); the product runs one thread per row.
+// The matrix is SELL-64 (mx_index); the product runs one thread per row.
 #include <type_traits>
 
 #include "vox_internal.hpp"
@@ -111,10 +111,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
     }
 }
 
-// One thread per row over the sliced layout: a wave's 64 rows read slot k as
-// one contiguous line set; 16 entries' loads are in flight per lane and the
-// row is accumulated in order (in VT). The matrix streams from HBM once per
-// product (non-temporal loads keep it from evicting the gathered p from L2).
+// Row products over the SELL-64 matrix, entries in row order after the
+// diagonal: acc = d_r p_r + sum_k a_rk p_col(rk). A wave's entry k of its 64
+// rows is one contiguous block; 16 entries' loads are in flight per lane. The
+// matrix streams from HBM once per product (non-temporal loads keep it from
+// evicting the gathered vector from L2).
+
+// q = A p and the partial p.q of each block; one thread per row.
 template <class VT, class MT>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
 {
@@ -123,11 +126,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
     double pq = 0.0;
     const int r = xcd_tid();
     if (r < a.nu) {
+        // (all 32 streaming loads of a step are issued before the first
+        // gather: keep this loop's shape, other forms scheduled the gathers
+        // earlier and measured about 18% slower)
         const VT pr = a.p[r];
         VT acc = (VT)a.diag[r] * pr;
         const int len = a.rowlen[r];
-        const int32_t* __restrict__ col = a.acol + sl_index(r, 0, a.cap);
-        const MT* __restrict__ val = a.aval + sl_index(r, 0, a.cap);
+        const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
+        const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
         constexpr int U = 16;
         int k = 0;
         for (; k + U <= len; k += U) {
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
     if (r < a.nu) {
         double acc = a.diag[r] * a.x[r];
         const int len = a.rowlen[r];
-        const int32_t* __restrict__ col = a.acol + sl_index(r, 0, a.cap);
-        const double* __restrict__ val = a.aval + sl_index(r, 0, a.cap);
+        const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
+        const double* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
         constexpr int U = 16;
         int k = 0;
         for (; k + U <= len; k += U) {

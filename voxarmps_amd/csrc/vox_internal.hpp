@@ -18,6 +18,7 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
 
+
 // Gather records: everything a neighbour loop reads about particle j in one
 // 32- or 64-byte aligned record (one cache line) instead of one line per array.
 struct __align__(32) PosRec {
@@ -64,6 +65,7 @@ enum DevFlag {
     FLAG_COLL_HITS,       // applied collision impulses
     FLAG_COLL_CAND,       // candidate pairs of the exact pass
     FLAG_COLL_VIOLATION,  // margin of the candidate screen exceeded
+    FLAG_COLL_BIG,        // a candidate component exceeds the per-thread replay bound
     FLAG_NU,              // number of PPE unknowns
     FLAG_CG_DONE,         // CG finished (converged, breakdown or cap)
     FLAG_CG_ITERS,
@@ -85,10 +87,10 @@ struct Grid {
 
 struct DevList {
     double re = 0.0;
-    int cap = 0;             // slots per row (slot-major storage)
+    int cap = 0;             // entries per row (multiple of 8)
     Grid g{};
     int32_t* cnt = nullptr;        // tp
-    int32_t* ids = nullptr;        // cap * tp, ids[l*tp + i]
+    int32_t* ids = nullptr;        // sliced rows, entry l of row i at sl_index(i, l, cap)
     int32_t* cell = nullptr;       // tp
     int32_t* cell_start = nullptr; // tnc + 1
     int32_t* cell_fill = nullptr;  // tnc
@@ -97,15 +99,68 @@ struct DevList {
     long tnc_alloc = 0;
 };
 
-// Sliced row storage (SELL-64): rows are grouped in slices of 64 (one
-// wavefront); inside a slice entry l of row i sits at
-//   ((i / 64) * cap + l) * 64 + i % 64,
-// so a wave reading slot l of its 64 rows touches one contiguous 256-byte
-// line set, and a row's successive slots are 64 entries apart (the whole row
-// of a slice spans cap*256 bytes instead of cap pages).
+// Sliced row storage in chunks of 8 (SELL-64x8). Rows are grouped in slices
+// of 64 (one wavefront) and cut into chunks of 8 consecutive entries; chunk g
+// of a slice's 64 rows is one contiguous block of 64 x 8 entries:
+//   entry l of row i sits at (((i/64) * (cap/8) + l/8) * 64 + i%64) * 8 + l%8.
+// A row's 8 consecutive entries are contiguous (one 32-byte sector of int32),
+// so a lane that appends to its own row fills whole sectors instead of leaving
+// one partly written sector per entry, and a wave reading chunk g of its 64
+// rows reads one contiguous block (2 KB of int32). cap is a multiple of 8.
+constexpr int kRowChunk = 8;
 __host__ __device__ __forceinline__ long sl_index(long i, int l, int cap)
 {
-    return ((i >> 6) * (long)cap + l) * 64 + (i & 63);
+    return ((((i >> 6) * (long)(cap >> 3) + (l >> 3)) * 64 + (i & 63)) << 3) + (l & 7);
+}
+// Offset of entry l from entry 0 of the same row.
+__host__ __device__ __forceinline__ long sl_step(int l) { return ((long)(l >> 3) << 9) + (l & 7); }
+inline int round_cap(int cap) { return (cap + kRowChunk - 1) & ~(kRowChunk - 1); }
+
+// Matrix rows (PPE) use plain SELL-64: entry l of row r at ((r/64)*cap + l)*64
+// + r%64, so a wave reading entry l of its 64 rows (the product's access) reads
+// one contiguous 256/512-byte block.
+__host__ __device__ __forceinline__ long mx_index(long r, int l, int cap)
+{
+    return ((r >> 6) * (long)cap + l) * 64 + (r & 63);
+}
+
+typedef int32_t v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Non-temporal loads of 8 consecutive entries (p 32-byte aligned): streamed
+// data that should not displace gathered vectors from the caches.
+__device__ __forceinline__ void load8_nt(const int32_t* __restrict__ p, int32_t (&o)[8])
+{
+    const v4i a = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(p));
+    const v4i b = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(p) + 1);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+}
+__device__ __forceinline__ void load8_nt(const double* __restrict__ p, double (&o)[8])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const v2d a = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p) + k);
+        o[2 * k] = a.x; o[2 * k + 1] = a.y;
+    }
+}
+__device__ __forceinline__ void load8_nt(const float* __restrict__ p, float (&o)[8])
+{
+    const v4f a = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p));
+    const v4f b = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(p) + 1);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+}
+
+// The 8 entries of chunk g of a row (row = address of the row's entry 0,
+// 32-byte aligned): two 16-byte loads.
+__device__ __forceinline__ void load_chunk8(const int32_t* __restrict__ row, int g, int32_t (&o)[8])
+{
+    const v4i* p = reinterpret_cast<const v4i*>(row + ((long)g << 9));
+    const v4i a = p[0], b = p[1];
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
 }
 inline long sl_size(long n, int cap) { return ((n + 63) / 64) * 64 * (long)cap; }
 
@@ -142,13 +197,13 @@ struct vox_ctx {
     vox::PredRec* pred_rec = nullptr;  // tp gather records of the predictor
     vox::PosRec* pos_rec = nullptr;    // tp gather records (positions + two ints)
 
-    // PPE (rows are unknowns r = uidx[i]; slot-major with stride tp)
+    // PPE (rows are unknowns r = uidx[i]; SELL-64 rows, mx_index)
     int32_t* uidx = nullptr;       // tp: particle -> unknown or -1
     int32_t* urow = nullptr;       // tp: unknown -> particle
     int32_t* uflag = nullptr;      // tp scan input
-    int32_t* rowlen = nullptr;     // tp
-    int32_t* acol = nullptr;       // cap_l * tp
-    double* aval = nullptr;        // cap_l * tp
+    int32_t* rowlen = nullptr;     // tp: stored off-diagonal entries of each row
+    int32_t* acol = nullptr;       // cap_l * tp, mx_index
+    double* aval = nullptr;        // cap_l * tp, mx_index
     double* adiag = nullptr;       // tp
     double* rhs = nullptr;         // tp
     int a_cap = 0;
@@ -177,6 +232,7 @@ struct vox_ctx {
     int32_t* coll_i = nullptr;     // cand capacity
     int32_t* coll_j = nullptr;
     int32_t* coll_stamp = nullptr; // tp
+    int32_t* coll_root = nullptr;  // tp: smallest particle of each candidate component
     int32_t* coll_logidx = nullptr;// tp
     double* coll_log = nullptr;    // 8 doubles per logged particle
     int32_t* coll_logn = nullptr;  // 1
@@ -212,7 +268,7 @@ namespace vox {
 enum KClass {
     KC_CELL = 0, KC_SORT, KC_LIST, KC_PND, KC_SPS, KC_VFRAC, KC_VISC, KC_PRED,
     KC_BCON, KC_ASSEMBLE, KC_CG_INIT, KC_SPMV, KC_CG_UPDATE, KC_CG_P, KC_PHAT,
-    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_IR, KC_COUNT
+    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_IR, KC_LIST_L, KC_COUNT
 };
 const char* kclass_name(int k);
 int kclass_from_name(const char* name);

@@ -17,7 +17,8 @@
 // Pipeline per list: key+count (atomic histogram) -> exclusive scan (cell
 // start) -> atomic scatter -> per-cell rank sort by id (one wave per cell,
 // LDS tile) with position records copied into cell order -> per-particle stencil
-// walk writing slot-major rows ids[l*tp + i].
+// walk (one thread per particle, or one wave per cell) writing sliced rows
+// (sl_index).
 #include "vox_internal.hpp"
 
 namespace vox {
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                                 acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
                             acc = acc && (j0 + u < e) && (g.dim == 3 || cu.id != i);
                             if (acc) {
-                                if (k < cap) ids[rb + 64L * k] = cu.id;
+                                if (k < cap) ids[rb + sl_step(k)] = cu.id;
                                 ++k;
                             }
                         }
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(kWave) void k_build_list_cells(
                                 acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
                             acc = acc && active && (g.dim == 3 || cu.id != i);
                             if (acc) {
-                                if (k < cap) ids[rb + 64L * k] = cu.id;
+                                if (k < cap) ids[rb + sl_step(k)] = cu.id;
                                 ++k;
                             }
                         }
@@ -420,6 +421,7 @@ __global__ void k_reduce_max(const int32_t* __restrict__ v, int n, int32_t* __re
 int ensure_list_storage(vox_ctx* c, int which, int cap)
 {
     DevList& L = c->list[which];
+    cap = round_cap(cap);
     if (cap <= L.cap && L.ids) return VOX_OK;
     if (L.ids) (void)hipFree(L.ids);
     L.ids = nullptr;
@@ -476,7 +478,7 @@ static int build_one(vox_ctx* c, int which, double re)
                            (c->p.list_mode == 0 && per_cell >= 24.0 && L.g.tnc * kCellHalves < (1L << 31));
     for (int attempt = 0; attempt < 4; ++attempt) {
         {
-            ProfScope ps(c, KC_LIST);
+            ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
             VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
             if (cell_walk) {
                 VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));

@@ -6,9 +6,9 @@
 // off-diagonal a_ij = -((2*dim/lambda)*W(d_ij)/n0/rho1) is kept for unknown
 // columns, free-surface (bcon == 1) neighbours only add to the diagonal and
 // dummy (bcon == -1) neighbours are skipped; a_ii = sum of |a_ij| in list order
-// plus 1e-7/dt/dt. Storage is sliced ELL (SELL-64, sl_index): a wave's slot k of
-// its 64 rows is one contiguous line set; only nonzero unknown columns are
-// stored. The solver lives in vox_cg.hip.
+// plus 1e-7/dt/dt. Storage is SELL-64 (mx_index): a wave's entry k of its 64
+// rows is one contiguous block, which is what the product reads; only nonzero
+// unknown columns are stored. The solver lives in vox_cg.hip.
 //
 #include "vox_internal.hpp"
 
@@ -67,22 +67,23 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     a.source[i] = src;
     if (bi != 0) return;
     const int r = a.uidx[i];
-    const long ob = sl_index(r, 0, a.acap);
-    const long lb = sl_index(i, 0, a.L.cap);
+    const long ob = mx_index(r, 0, a.acap);
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
     const PosRec me = a.rec[i];
     const double xi = me.x, yi = me.y, zi = me.z;
     double diag = 0.0;
     int k = 0;
     const int cnt = a.L.cnt[i];
-    // Chunks of 8 slots: all index and record loads of a chunk are issued
-    // before its stores (the stores would otherwise order every next load
-    // behind them); entries are then processed strictly in row order.
+    // One row chunk of 8 per step: all index and record loads of a chunk are
+    // issued before its stores (the stores would otherwise order every next
+    // load behind them); entries are then processed strictly in row order.
     constexpr int U = 8;
     for (int l0 = 0; l0 < cnt; l0 += U) {
         int jj[U];
         PosRec rr[U];
+        load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? a.L.ids[lb + 64L * (l0 + u)] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) rr[u] = a.rec[jj[u]];
         // Branch-free per entry (a skipped entry keeps diag unchanged through a
@@ -113,14 +114,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 }
 
 // FP32 copy of the off-diagonal values for the mixed-precision inner solves:
-// one thread per row, so a wave's 64 rows read and write slot k of their
-// slice as one contiguous line set.
+// one thread per row, so a wave's 64 rows read and write entry k of their
+// slice as one contiguous block.
 __global__ void k_aval32(int nu, int cap, const int32_t* __restrict__ rowlen,
                          const double* __restrict__ aval, float* __restrict__ aval32)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nu) return;
-    const long ob = sl_index(r, 0, cap);
+    const long ob = mx_index(r, 0, cap);
     const int len = rowlen[r];
     constexpr int U = 8;
     int k = 0;
@@ -168,7 +169,8 @@ int ppe_assemble(vox_ctx* c)
     a.acap = c->a_cap;
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z]; a.pnd = c->f[VOX_F_N];
     a.bcon = c->bcon; a.uidx = c->uidx;
-    a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval; a.adiag = c->adiag; a.rhs = c->rhs;
+    a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
+    a.adiag = c->adiag; a.rhs = c->rhs;
     a.source = c->f[VOX_F_SOURCE];
     a.rec = c->pos_rec;
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
@@ -184,11 +186,12 @@ int ppe_assemble(vox_ctx* c)
     return VOX_OK;
 }
 
+// CSR over the unknowns: the diagonal first, then the stored entries in list
+// order.
 int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, int32_t* cols,
                double* vals, double* rhs, int32_t* ids)
 {
     const int nu = c->nu;
-    const long n = c->tp;
     std::vector<int32_t> len(nu > 0 ? nu : 1);
     if (nu > 0)
         VOX_HIP(hipMemcpy(len.data(), c->rowlen, sizeof(int32_t) * nu, hipMemcpyDeviceToHost));
@@ -216,8 +219,9 @@ int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, i
         if (vals) vals[k] = diag[r];
         ++k;
         for (int s = 0; s < len[r]; ++s) {
-            if (cols) cols[k] = acol[sl_index(r, s, c->a_cap)];
-            if (vals) vals[k] = aval[sl_index(r, s, c->a_cap)];
+            const double v = aval[mx_index(r, s, c->a_cap)];
+            if (cols) cols[k] = acol[mx_index(r, s, c->a_cap)];
+            if (vals) vals[k] = v;
             ++k;
         }
         if (rhs) rhs[r] = b[r];

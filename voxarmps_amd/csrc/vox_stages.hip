@@ -1,15 +1,13 @@
 // vox_stages.hip — per-particle stages of the MPS time step on gfx950.
 //
-// One thread per particle; neighbour rows are slot-major (ids[l*tp + i]) so a
-// wave's 64 index loads per slot are one coalesced 256-byte request, and each
-// row is summed sequentially in the reference's order so that, built with
-// -ffp-contract=off, every per-particle sum is bit-identical to the reference's
-// sequential CPU path.
+// One thread per particle; neighbour rows are read 8 entries at a time from the
+// sliced row storage (one 32-byte load pair per lane, one contiguous block per
+// wave), and each row is summed sequentially in the reference's order so that,
+// built with -ffp-contract=off, every per-particle sum is bit-identical to the
+// reference's sequential CPU path.
 #include "vox_internal.hpp"
 
 namespace vox {
-
-#define VOX_ROW_J(L, l, i) ((L).ids[sl_index((i), (l), (L).cap)])
 
 __device__ __forceinline__ double pdist(int dim, const double* __restrict__ x,
                                         const double* __restrict__ y,
@@ -56,11 +54,18 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, doubl
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     double sum = 0.0;
     int cnt = L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(L, l, i);
-        if (j == i) continue;
-        double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
-        sum = sum + wfun(d, ktype, re);
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            int j = jc[e];
+            if (j == i) continue;
+            double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
+            sum = sum + wfun(d, ktype, re);
+        }
     }
     if (ktype != 2) sum = sum * ncorr;
     out[i] = sum;
@@ -126,26 +131,33 @@ __device__ __forceinline__ Strain strain_row(int i, int dim, int ktype, double r
     double ui = u[i], vi = v[i], wi = (dim == 3) ? w[i] : 0.0;
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
     int cnt = L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(L, l, i);
-        if (j == i) continue;
-        double dx = x[j] - xi, dy = y[j] - yi, dz = 0.0, d;
-        if (dim == 3) { dz = z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
-        else d = sqrt(dx * dx + dy * dy);
-        if (use_dmin && !(d > dmin)) continue;
-        double wv = wfun(d, ktype, re);
-        double du = u[j] - ui, dv = v[j] - vi;
-        s1 = s1 + du * dx * wv / d / d;
-        s2 = s2 + dv * dy * wv / d / d;
-        s3 = s3 + du * dy * wv / d / d;
-        s4 = s4 + dv * dx * wv / d / d;
-        if (dim == 3) {
-            double dw = w[j] - wi;
-            s5 = s5 + dw * dz * wv / d / d;
-            s6 = s6 + du * dz * wv / d / d;
-            s7 = s7 + dv * dz * wv / d / d;
-            s8 = s8 + dw * dx * wv / d / d;
-            s9 = s9 + dw * dy * wv / d / d;
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            int j = jc[e];
+            if (j == i) continue;
+            double dx = x[j] - xi, dy = y[j] - yi, dz = 0.0, d;
+            if (dim == 3) { dz = z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
+            else d = sqrt(dx * dx + dy * dy);
+            if (use_dmin && !(d > dmin)) continue;
+            double wv = wfun(d, ktype, re);
+            double du = u[j] - ui, dv = v[j] - vi;
+            s1 = s1 + du * dx * wv / d / d;
+            s2 = s2 + dv * dy * wv / d / d;
+            s3 = s3 + du * dy * wv / d / d;
+            s4 = s4 + dv * dx * wv / d / d;
+            if (dim == 3) {
+                double dw = w[j] - wi;
+                s5 = s5 + dw * dz * wv / d / d;
+                s6 = s6 + du * dz * wv / d / d;
+                s7 = s7 + dv * dz * wv / d / d;
+                s8 = s8 + dw * dx * wv / d / d;
+                s9 = s9 + dw * dy * wv / d / d;
+            }
         }
     }
     double Uxx = (2.0 / n0) * s1, Uyy = (2.0 / n0) * s2, Uxy = (2.0 / n0) * s3, Uyx = (2.0 / n0) * s4;
@@ -194,18 +206,25 @@ __global__ __launch_bounds__(kBlock) void k_sps2(SpsArgs a)
     double xi = a.x[i], yi = a.y[i], zi = (a.dim == 3) ? a.z[i] : 0.0, ni = a.neut[i];
     double s1 = 0, s2 = 0, s3 = 0;
     int cnt = a.L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(a.L, l, i);
-        if (j == i) continue;
-        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, d;
-        if (a.dim == 3) { dz = a.z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
-        else d = sqrt(dx * dx + dy * dy);
-        if (!(d > a.dmin)) continue;
-        double wv = wfun(d, a.ktype, a.re);
-        double dn = a.neut[j] - ni;
-        s1 = s1 + dn * dx * wv / d / d;
-        s2 = s2 + dn * dy * wv / d / d;
-        if (a.dim == 3) s3 = s3 + dn * dz * wv / d / d;
+    const int32_t* __restrict__ rowp = a.L.ids + sl_index(i, 0, a.L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            int j = jc[e];
+            if (j == i) continue;
+            double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, d;
+            if (a.dim == 3) { dz = a.z[j] - zi; d = sqrt(dx * dx + dy * dy + dz * dz); }
+            else d = sqrt(dx * dx + dy * dy);
+            if (!(d > a.dmin)) continue;
+            double wv = wfun(d, a.ktype, a.re);
+            double dn = a.neut[j] - ni;
+            s1 = s1 + dn * dx * wv / d / d;
+            s2 = s2 + dn * dy * wv / d / d;
+            if (a.dim == 3) s3 = s3 + dn * dz * wv / d / d;
+        }
     }
     double s11 = a.s11[i], s12 = a.s12[i], s22 = a.s22[i];
     if (a.dim == 3) {
@@ -254,14 +273,21 @@ __global__ __launch_bounds__(kBlock) void k_vfraction(int n, int dim, int ktype,
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     double s1 = 0, s2 = 0;
     int cnt = L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(L, l, i);
-        if (j == i) continue;
-        int tj = pt[j];
-        if (tj <= 0) continue;
-        double wv = wfun(pdist(dim, x, y, z, i, j, xi, yi, zi), ktype, re);
-        s1 = s1 + wv;
-        if (tj == 2) s2 = s2 + wv;
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            int j = jc[e];
+            if (j == i) continue;
+            int tj = pt[j];
+            if (tj <= 0) continue;
+            double wv = wfun(pdist(dim, x, y, z, i, j, xi, yi, zi), ktype, re);
+            s1 = s1 + wv;
+            if (tj == 2) s2 = s2 + wv;
+        }
     }
     double cv = s2 / s1;
     if (s1 == 0) cv = 0;
@@ -383,20 +409,25 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
     const int cnt = a.L.cnt[i];
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
-    // chunks of 4 neighbours: their index, record and type loads are issued
-    // together, then the terms are accumulated in row order
+    // One row chunk (8 ids) per outer step, consumed in halves of 4: a half's
+    // record and type loads are issued together, then its terms are
+    // accumulated in row order.
+    for (int g = 0; (g << 3) < cnt; ++g) {
+    int32_t jc[8];
+    load_chunk8(row, g, jc);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
     constexpr int U = 4;
-    for (int l0 = 0; l0 < cnt; l0 += U) {
     int jj[U], tj[U];
     PredRec rv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + 4 * h + u < cnt) ? jc[4 * h + u] : i;
 #pragma unroll
     for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
-#pragma unroll
     // Branch-free per neighbour: every term is computed and a skipped entry
     // (self, padding, coincident particle) keeps each sum through a select, so
     // no gathered field is sunk into a branch behind a full memory wait.
+#pragma unroll
     for (int u = 0; u < U; ++u) {
         const PredRec rj = rv[u];
         // z, w are 0 in 2D records: the 3-term distance equals the 2D one
@@ -420,6 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
         s3 = use ? s3 + dp * dz * wg / D / D : s3;   // z sums: only read in 3D
         s6 = use ? s6 + wl * dw * NEU : s6;
         s9 = use ? s9 + wl * dw : s9;
+    }
     }
     }
     double nt = a.neut[i];
@@ -515,9 +547,16 @@ __global__ void k_phat(int n, ListView L, const double* __restrict__ pn, double*
     if (i >= n) return;
     double mn = 999999999999999.0;
     int cnt = L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        double v = pn[VOX_ROW_J(L, l, i)];
-        if (v < mn) mn = v;
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            double v = pn[jc[e]];
+            if (v < mn) mn = v;
+        }
     }
     ph[i] = mn;
 }
@@ -549,17 +588,24 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
     double pi_ = a.pn[i], phi_ = a.ph[i];
     double s1 = 0, s2 = 0, s3 = 0;
     int cnt = a.L.cnt[i];
-    for (int l = 0; l < cnt; ++l) {
-        int j = VOX_ROW_J(a.L, l, i);
-        if (j == i) continue;
-        double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
-        if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
-        else D = sqrt(dx * dx + dy * dy);
-        double wv = wfun(D, a.ktype, a.re);
-        double dp = (a.kh == 1) ? (a.pn[j] + pi_ - (phi_ + a.ph[j])) : (a.pn[j] - phi_);
-        s1 = s1 + dp * dx * wv / D / D;
-        s2 = s2 + dp * dy * wv / D / D;
-        if (dim == 3) s3 = s3 + dp * dz * wv / D / D;
+    const int32_t* __restrict__ rowp = a.L.ids + sl_index(i, 0, a.L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= cnt) break;
+            int j = jc[e];
+            if (j == i) continue;
+            double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
+            if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
+            else D = sqrt(dx * dx + dy * dy);
+            double wv = wfun(D, a.ktype, a.re);
+            double dp = (a.kh == 1) ? (a.pn[j] + pi_ - (phi_ + a.ph[j])) : (a.pn[j] - phi_);
+            s1 = s1 + dp * dx * wv / D / D;
+            s2 = s2 + dp * dy * wv / D / D;
+            if (dim == 3) s3 = s3 + dp * dz * wv / D / D;
+        }
     }
     double g = a.relaxp * (a.gnum * a.dt / a.n0 / a.rho[i]);
     double un = a.us[i] - g * s1;
@@ -654,12 +700,13 @@ int st_euler(vox_ctx* c)
 // The reference updates particles i and j in place while it walks i = 1..TP and
 // each row in order, so a later pair sees earlier updates. That order is
 // reproduced exactly: a parallel screen lists every (i, j) with d < coll+margin
-// in (i, slot) order; one lane then replays exactly the reference loop over
-// those candidates with the live positions. A pair outside the screen can only
-// reach d < coll if its particles move by more than margin in total during the
-// pass; the replay accumulates each touched particle's path length and raises
-// a flag if any exceeds margin/2 — then an undo log restores the state and the
-// pass repeats with a wider screen.
+// in (i, slot) order; the candidates are then replayed in the reference order
+// with the live positions, one thread per connected component of the candidate
+// pairs (components share no particle, so they are independent). A pair
+// outside the screen can only reach d < coll if its particles move by more
+// than margin in total during the pass; the replay accumulates each touched
+// particle's path length and raises a flag if any exceeds margin/2 — then an
+// undo log restores the state and the pass repeats with a wider screen.
 __global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double rc, ListView L,
                                                         const double* __restrict__ x,
                                                         const double* __restrict__ y,
@@ -675,13 +722,20 @@ __global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double r
     long pos = off ? off[i] : 0;
     int k = 0;
     int rows = L.cnt[i];
-    for (int l = 0; l < rows; ++l) {
-        int j = VOX_ROW_J(L, l, i);
-        if (j == i) continue;
-        double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
-        if (!(d < rc)) continue;
-        if (off && pos + k < cap) { ci[pos + k] = i; cj[pos + k] = j; }
-        ++k;
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < rows; ++g) {
+        int32_t jc[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            if ((g << 3) + e >= rows) break;
+            int j = jc[e];
+            if (j == i) continue;
+            double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
+            if (!(d < rc)) continue;
+            if (off && pos + k < cap) { ci[pos + k] = i; cj[pos + k] = j; }
+            ++k;
+        }
     }
     if (!off) cnt[i] = k;
 }
@@ -698,78 +752,172 @@ struct CollArgs {
     int32_t epoch;
 };
 
+// Applies candidate k exactly as the reference loop body (functions.cu:1540-1595)
+// with the live positions; logs each particle the first time it moves (for the
+// undo) and accumulates its path length. Returns 1 if an impulse was applied.
+__device__ __forceinline__ int coll_apply(const CollArgs& a, int k)
+{
+    const int dim = a.dim;
+    const int i = a.ci[k], j = a.cj[k];
+    const int ti = a.pt[i], tj = a.pt[j];
+    const double m1 = (ti == 1) ? a.rho1 : a.rho2;
+    const double m2 = (tj == 1) ? a.rho1 : a.rho2;
+    double dx = a.x[j] - a.x[i], dy = a.y[j] - a.y[i], dz = 0.0, d;
+    if (dim == 3) { dz = a.z[j] - a.z[i]; d = sqrt(dx * dx + dy * dy + dz * dz); }
+    else d = sqrt(dx * dx + dy * dy);
+    if (!(d < a.coll)) return 0;
+    double ug = (m1 * a.u[i] + m2 * a.u[j]) / (m1 + m2);
+    double vg = (m1 * a.v[i] + m2 * a.v[j]) / (m1 + m2);
+    double wg = 0, wr = 0, wm = 0;
+    if (dim == 3) wg = (m1 * a.w[i] + m2 * a.w[j]) / (m1 + m2);
+    double ur = m1 * (a.u[i] - ug);
+    double vr = m1 * (a.v[i] - vg);
+    if (dim == 3) wr = m1 * (a.w[i] - wg);
+    double vabs = (dim == 3) ? (ur * dx + vr * dy + wr * dz) / d : (ur * dx + vr * dy) / d;
+    double um = (1.0 + a.cc) * vabs * dx / d;
+    double vm = (1.0 + a.cc) * vabs * dy / d;
+    if (dim == 3) wm = (1.0 + a.cc) * vabs * dz / d;
+    if (!(vabs > 0)) return 0;
+    for (int side = 0; side < 2; ++side) {
+        const int q = side ? j : i;
+        if ((side ? tj : ti) <= 0) continue;
+        if (a.stamp[q] != a.epoch) {
+            a.stamp[q] = a.epoch;
+            const int slot = atomicAdd(a.logn, 1);
+            a.logidx[q] = slot;
+            double* e = a.log + 8L * slot;
+            e[0] = (double)q; e[1] = a.x[q]; e[2] = a.y[q]; e[3] = a.u[q]; e[4] = a.v[q];
+            e[5] = (dim == 3) ? a.z[q] : 0.0; e[6] = (dim == 3) ? a.w[q] : 0.0; e[7] = 0.0;
+        }
+        const double m = side ? m2 : m1;
+        double nx, ny, nz = 0.0;
+        if (!side) {
+            a.u[q] = a.u[q] - um / m;
+            a.v[q] = a.v[q] - vm / m;
+            nx = a.x[q] - a.dt * um / m;
+            ny = a.y[q] - a.dt * vm / m;
+        } else {
+            a.u[q] = a.u[q] + um / m;
+            a.v[q] = a.v[q] + vm / m;
+            nx = a.x[q] + a.dt * um / m;
+            ny = a.y[q] + a.dt * vm / m;
+        }
+        double step = fabs(nx - a.x[q]) + fabs(ny - a.y[q]);
+        a.x[q] = nx; a.y[q] = ny;
+        if (dim == 3) {
+            if (!side) { a.w[q] = a.w[q] - wm / m; nz = a.z[q] - a.dt * wm / m; }
+            else { a.w[q] = a.w[q] + wm / m; nz = a.z[q] + a.dt * wm / m; }
+            step += fabs(nz - a.z[q]);
+            a.z[q] = nz;
+        }
+        a.log[8L * a.logidx[q] + 7] += step;
+    }
+    return 1;
+}
+
+// Single-lane replay of every candidate in (i, slot) order.
 __global__ void k_coll_replay(CollArgs a)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const int total = *a.total;
-    int nlog = 0, hits = 0;
-    const int dim = a.dim;
-    for (int k = 0; k < total && k < a.cap; ++k) {
-        const int i = a.ci[k], j = a.cj[k];
-        const int ti = a.pt[i], tj = a.pt[j];
-        const double m1 = (ti == 1) ? a.rho1 : a.rho2;
-        const double m2 = (tj == 1) ? a.rho1 : a.rho2;
-        double dx = a.x[j] - a.x[i], dy = a.y[j] - a.y[i], dz = 0.0, d;
-        if (dim == 3) { dz = a.z[j] - a.z[i]; d = sqrt(dx * dx + dy * dy + dz * dz); }
-        else d = sqrt(dx * dx + dy * dy);
-        if (!(d < a.coll)) continue;
-        double ug = (m1 * a.u[i] + m2 * a.u[j]) / (m1 + m2);
-        double vg = (m1 * a.v[i] + m2 * a.v[j]) / (m1 + m2);
-        double wg = 0, wr = 0, wm = 0;
-        if (dim == 3) wg = (m1 * a.w[i] + m2 * a.w[j]) / (m1 + m2);
-        double ur = m1 * (a.u[i] - ug);
-        double vr = m1 * (a.v[i] - vg);
-        if (dim == 3) wr = m1 * (a.w[i] - wg);
-        double vabs = (dim == 3) ? (ur * dx + vr * dy + wr * dz) / d : (ur * dx + vr * dy) / d;
-        double um = (1.0 + a.cc) * vabs * dx / d;
-        double vm = (1.0 + a.cc) * vabs * dy / d;
-        if (dim == 3) wm = (1.0 + a.cc) * vabs * dz / d;
-        if (!(vabs > 0)) continue;
-        ++hits;
-        for (int side = 0; side < 2; ++side) {
-            const int q = side ? j : i;
-            if ((side ? tj : ti) <= 0) continue;
-            if (a.stamp[q] != a.epoch) {
-                a.stamp[q] = a.epoch;
-                a.logidx[q] = nlog;
-                double* e = a.log + 8L * nlog;
-                e[0] = (double)q; e[1] = a.x[q]; e[2] = a.y[q]; e[3] = a.u[q]; e[4] = a.v[q];
-                e[5] = (dim == 3) ? a.z[q] : 0.0; e[6] = (dim == 3) ? a.w[q] : 0.0; e[7] = 0.0;
-                ++nlog;
-            }
-            const double m = side ? m2 : m1;
-            const double sg = side ? 1.0 : -1.0;
-            double nx, ny, nz = 0.0;
-            if (!side) {
-                a.u[q] = a.u[q] - um / m;
-                a.v[q] = a.v[q] - vm / m;
-                nx = a.x[q] - a.dt * um / m;
-                ny = a.y[q] - a.dt * vm / m;
-            } else {
-                a.u[q] = a.u[q] + um / m;
-                a.v[q] = a.v[q] + vm / m;
-                nx = a.x[q] + a.dt * um / m;
-                ny = a.y[q] + a.dt * vm / m;
-            }
-            double step = fabs(nx - a.x[q]) + fabs(ny - a.y[q]);
-            a.x[q] = nx; a.y[q] = ny;
-            if (dim == 3) {
-                if (!side) { a.w[q] = a.w[q] - wm / m; nz = a.z[q] - a.dt * wm / m; }
-                else { a.w[q] = a.w[q] + wm / m; nz = a.z[q] + a.dt * wm / m; }
-                step += fabs(nz - a.z[q]);
-                a.z[q] = nz;
-            }
-            (void)sg;
-            a.log[8L * a.logidx[q] + 7] += step;
+    int hits = 0;
+    for (int k = 0; k < total && k < a.cap; ++k) hits += coll_apply(a, k);
+    a.flags[FLAG_COLL_HITS] = hits;
+}
+
+// Candidate components. Candidate pairs are listed in both orders (the screen
+// is symmetric), so the candidates of particle p are ci == p at off[p] ..
+// off[p] + cnt[p] and its partners are their cj. Particles joined through
+// candidate pairs form components; a candidate only reads and moves its two
+// particles, so the sequential replay of one component is independent of every
+// other component, and the reference order restricted to a component is its
+// members in ascending id, each with its own candidates in slot order.
+constexpr int kCompMax = 48;   // members one thread collects
+
+// Members of p's component in ascending id (at most kCompMax; returns -1 if
+// the component is larger).
+__device__ int coll_component(int p, const int32_t* __restrict__ off,
+                              const int32_t* __restrict__ cnt, const int32_t* __restrict__ cj,
+                              int* mem)
+{
+    int m = 1, head = 0;
+    mem[0] = p;
+    while (head < m) {
+        const int q = mem[head++];
+        const int s = off[q], e = s + cnt[q];
+        for (int k = s; k < e; ++k) {
+            const int r = cj[k];
+            bool seen = false;
+            for (int t = 0; t < m; ++t) seen = seen || (mem[t] == r);
+            if (seen) continue;
+            if (m == kCompMax) return -1;
+            mem[m++] = r;
         }
     }
-    int viol = 0;
-    for (int k = 0; k < nlog; ++k)
-        if (a.log[8L * k + 7] > a.half_margin) viol = 1;
-    a.flags[FLAG_COLL_HITS] = hits;
-    a.flags[FLAG_COLL_CAND] = total;
-    a.flags[FLAG_COLL_VIOLATION] = viol;
-    *a.logn = nlog;
+    for (int t = 1; t < m; ++t) {            // insertion sort, ascending id
+        const int v = mem[t];
+        int u = t - 1;
+        while (u >= 0 && mem[u] > v) { mem[u + 1] = mem[u]; --u; }
+        mem[u + 1] = v;
+    }
+    return m;
+}
+
+// Pass 1: the smallest member of each particle's component; flags components
+// beyond the per-thread bound.
+__global__ __launch_bounds__(kBlock) void k_coll_roots(int n, const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ cnt,
+                                                       const int32_t* __restrict__ cj,
+                                                       int32_t* __restrict__ root,
+                                                       int32_t* __restrict__ flags)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (cnt[p] == 0) { root[p] = -2; return; }   // never equal to a candidate end's root
+    int mem[kCompMax];
+    const int m = coll_component(p, off, cnt, cj, mem);
+    if (m < 0) { atomicMax(&flags[FLAG_COLL_BIG], 1); root[p] = -1; return; }
+    root[p] = mem[0];
+}
+
+// The component construction relies on every candidate pair being listed in
+// both orders (symmetric lists); a pair whose two ends found different
+// components means it was not, and the replay falls back to a single lane.
+__global__ void k_coll_verify(const int32_t* __restrict__ total, const int32_t* __restrict__ ci,
+                              const int32_t* __restrict__ cj, const int32_t* __restrict__ root,
+                              int32_t* __restrict__ flags)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *total) return;
+    if (root[ci[k]] != root[cj[k]]) flags[FLAG_COLL_BIG] = 2;
+}
+
+// Pass 2: one thread per component (its smallest member) replays the
+// component's candidates in reference order.
+__global__ __launch_bounds__(kBlock) void k_coll_replay_comp(CollArgs a, int n,
+                                                             const int32_t* __restrict__ off,
+                                                             const int32_t* __restrict__ cnt,
+                                                             const int32_t* __restrict__ root)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || cnt[p] == 0 || root[p] != p) return;
+    int mem[kCompMax];
+    const int m = coll_component(p, off, cnt, a.cj, mem);
+    if (m < 0) return;                       // not reached: pass 1 flagged it
+    int hits = 0;
+    for (int t = 0; t < m; ++t) {
+        const int q = mem[t];
+        for (int k = off[q], e = off[q] + cnt[q]; k < e; ++k) hits += coll_apply(a, k);
+    }
+    if (hits) atomicAdd(&a.flags[FLAG_COLL_HITS], hits);
+}
+
+// Any touched particle that moved more than half the screen margin.
+__global__ void k_coll_check(CollArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *a.logn) return;
+    if (a.log[8L * k + 7] > a.half_margin) a.flags[FLAG_COLL_VIOLATION] = 1;
 }
 
 __global__ void k_coll_undo(int dim, const int32_t* __restrict__ logn, const double* __restrict__ log,
@@ -799,6 +947,30 @@ static int coll_reserve(vox_ctx* c, long need)
     return VOX_OK;
 }
 
+__global__ void k_coll_reset(int32_t* __restrict__ flags, int32_t* __restrict__ logn)
+{
+    if (threadIdx.x == 0) {
+        flags[FLAG_COLL_HITS] = 0;
+        flags[FLAG_COLL_VIOLATION] = 0;
+        flags[FLAG_COLL_BIG] = 0;
+        *logn = 0;
+    }
+}
+
+// Single-lane replay of the candidates of the components too large for one
+// thread's member list (root == -1), in (i, slot) order.
+__global__ void k_coll_replay_big(CollArgs a, const int32_t* __restrict__ root)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int total = *a.total;
+    int hits = 0;
+    for (int k = 0; k < total && k < a.cap; ++k)
+        if (root[a.ci[k]] < 0) hits += coll_apply(a, k);
+    a.flags[FLAG_COLL_HITS] += hits;
+}
+
+constexpr double kCollMarginBase = 0.1;     // screen margin, fraction of coll
+
 int st_collision(vox_ctx* c)
 {
     ProfScope ps(c, KC_COLL);
@@ -807,7 +979,10 @@ int st_collision(vox_ctx* c)
     double* X = c->f[VOX_F_X]; double* Y = c->f[VOX_F_Y]; double* Z = c->f[VOX_F_Z];
     double* U = c->f[VOX_F_UNEW]; double* V = c->f[VOX_F_VNEW]; double* W = c->f[VOX_F_WNEW];
     ListView L = view(c, VOX_LIST_S);
-    for (int attempt = 0; attempt < 6; ++attempt) {
+    // The margin widens on a violation and relaxes back towards its base over
+    // the following steps.
+    c->coll_margin = std::max(kCollMarginBase, 0.75 * c->coll_margin);
+    for (int attempt = 0; attempt < 8; ++attempt) {
         double margin = p.coll * c->coll_margin;
         double rc = p.coll + margin;
         k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, nullptr,
@@ -838,16 +1013,33 @@ int st_collision(vox_ctx* c)
         a.x = X; a.y = Y; a.z = Z; a.u = U; a.v = V; a.w = W;
         a.stamp = c->coll_stamp; a.logidx = c->coll_logidx; a.logn = c->coll_logn;
         a.flags = c->dflags; a.log = c->coll_log; a.epoch = c->coll_epoch;
-        k_coll_replay<<<1, kWave, 0, c->stream>>>(a);
+        k_coll_reset<<<1, kWave, 0, c->stream>>>(c->dflags, c->coll_logn);
+        k_coll_roots<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->coll_off, c->coll_cnt, c->coll_j,
+                                                             c->coll_root, c->dflags);
+        k_coll_verify<<<grid_for(total), kBlock, 0, c->stream>>>(c->coll_off + n, c->coll_i,
+                                                                 c->coll_j, c->coll_root, c->dflags);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(read_flags(c));
+        if (c->hflags[FLAG_COLL_BIG] == 2) {
+            k_coll_replay<<<1, kWave, 0, c->stream>>>(a);          // every candidate, one lane
+        } else {
+            k_coll_replay_comp<<<grid_for(n), kBlock, 0, c->stream>>>(a, n, c->coll_off,
+                                                                       c->coll_cnt, c->coll_root);
+            if (c->hflags[FLAG_COLL_BIG])
+                k_coll_replay_big<<<1, kWave, 0, c->stream>>>(a, c->coll_root);
+        }
+        VOX_HIP(hipGetLastError());
+        // at most two log entries per candidate
+        k_coll_check<<<grid_for(2L * total), kBlock, 0, c->stream>>>(a);
         VOX_HIP(hipGetLastError());
         VOX_TRY(read_flags(c));
         c->st.collisions = c->hflags[FLAG_COLL_HITS];
         if (!c->hflags[FLAG_COLL_VIOLATION]) return VOX_OK;
         // undo and widen the screen
-        k_coll_undo<<<grid_for(2 * total), kBlock, 0, c->stream>>>(p.dim, c->coll_logn, c->coll_log,
-                                                                  X, Y, Z, U, V, W);
+        k_coll_undo<<<grid_for(2L * total), kBlock, 0, c->stream>>>(p.dim, c->coll_logn, c->coll_log,
+                                                                   X, Y, Z, U, V, W);
         VOX_HIP(hipGetLastError());
-        c->coll_margin *= 4.0;
+        c->coll_margin *= 2.0;
     }
     return fail(VOX_ERR_SOLVER, "collision screen could not bound particle motion");
 }

@@ -118,8 +118,9 @@ __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
     for (int l0 = 0; l0 < cnt; l0 += U) {
         int jj[U], tj[U];
         Pred32 rv[U];
+        load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
 #pragma unroll
@@ -229,8 +230,9 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad32(Grad32Args a)
     for (int l0 = 0; l0 < cnt; l0 += U) {
         int jj[U];
         Grad32 rv[U];
+        load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? row[64L * (l0 + u)] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) rv[u] = a.rec[jj[u]];
 #pragma unroll
