@@ -24,6 +24,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the product kernel measured with rocprofv3 --pmc
+# (FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy); see the file for the
+# commands. Used when its workload matches the one measured here.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_cg_spmv_dam3d_1175k.json")
+
+
+def measured_traffic(precision, nu, nnz):
+    """HBM bytes per k_cg_spmv launch from the committed counter summary, or
+    None when no summary matches this workload (same unknown and nonzero
+    counts within 1%)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get(precision)
+    if not e or abs(e["unknowns"] - nu) > 0.01 * nu or abs(e["nnz"] - nnz) > 0.01 * nnz:
+        return None
+    return e["bytes_per_launch"]
 
 
 def spmv_bytes(nu, nnz_total, vbytes=8):
@@ -132,9 +151,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=1, help="linear down-scaling of the workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+    ap.add_argument("--workload", choices=("dam3d", "ensemble"), default="dam3d",
+                    help="dam3d: the 3D dam break (1,175,450 particles); ensemble: 64 uneven 2D "
+                         "two-phase dam breaks batched in one context (mixed precision, SPS, "
+                         "Bingham fluid 2, neighbour-search walks chosen by the autotuner)")
+    ap.add_argument("--compare-solo", action="store_true",
+                    help="ensemble only: also run every scenario alone for the same steps and "
+                         "report the summed time")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default=None,
                     help="fp64: every kernel in FP64 (the reference's arithmetic); mixed: FP32 "
-                         "velocity stages and FP32 inner CG inside FP64 iterative refinement")
+                         "velocity stages and FP32 inner CG inside FP64 iterative refinement "
+                         "(default: fp64 for dam3d, mixed for ensemble)")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
     ap.add_argument("--precision-report", type=str, default="", metavar="STEPS",
@@ -163,14 +190,25 @@ def main():
     if nd.value < 1:
         raise SystemExit("no HIP device visible")
 
-    case = vx.dam_break_3d_c1(scale=args.scale)
-    p = vx.dam3d_params(case)
-    mixed = args.precision == "mixed"
-    if mixed:
-        p.precision = vx._abi.PREC_MIXED
+    ensemble = args.workload == "ensemble"
+    if ensemble:
+        case, region, boxes, scen = vx.ensemble_c4()
+        p = vx.ensemble_params(case)
+        precision = args.precision or "mixed"
+    else:
+        case = vx.dam_break_3d_c1(scale=args.scale)
+        p = vx.dam3d_params(case)
+        precision = args.precision or "fp64"
+    mixed = precision == "mixed"
+    p.precision = vx._abi.PREC_MIXED if mixed else vx._abi.PREC_FP64
     p.warm_start = 1 if args.warm_start else 0
     sim = vx.Simulation(p, case, backend=be)
+    modes = None
+    if ensemble:
+        sim.set_regions(region, boxes)
     sim.init()
+    if ensemble:
+        modes = sim.autotune(reps=3)
     print(f"# init done: {case.tp} particles", file=sys.stderr, flush=True)
     if args.warmup:
         sim.step(args.warmup)
@@ -209,10 +247,39 @@ def main():
         per_kernel[k] = round(t.value / max(args.steps, 1), 4)
     sim.close()
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    solo = None
+    if ensemble and args.compare_solo:
+        tot = 0.0
+        for k, s in enumerate(scen):
+            ps = vx.ensemble_params(s, box=boxes[k])
+            ps.precision = p.precision
+            a = vx.Simulation(ps, s, backend=be)
+            a.init()
+            a.step(1)
+            be.lib.vox_sync(a.ctx)
+            t1 = time.perf_counter()
+            a.step(args.steps)
+            be.lib.vox_sync(a.ctx)
+            tot += time.perf_counter() - t1
+            a.close()
+        solo = {"scenarios": len(scen), "ms_per_step_sum": tot / args.steps * 1e3,
+                "value": case.tp * args.steps / tot}
+        print("# solo runs done", file=sys.stderr, flush=True)
+
+    cpu = None if (args.no_cpu_baseline or ensemble) else cpu_baseline()
     value = case.tp * args.steps / el
+    if ensemble:
+        metric = "particle-steps/s, 64-scenario 2D two-phase ensemble"
+        workload = ("64 uneven 2D two-phase dam breaks in one context (columns 20-200 x 40-200 "
+                    "particles, l0=0.01 m), fluid 2 Bingham 1 Pa, SPS-LES Cs 0.12, " +
+                    ("mixed FP32/FP64" if mixed else "FP64"))
+    else:
+        metric = "particle-steps/s at 3D 1M"
+        workload = ("3D dam break, plain MPS, " + ("mixed FP32/FP64" if mixed else "FP64") +
+                    ", l0=0.005 m, water 100x100x80 in 1.6x0.6x0.405 m tank, "
+                    "1 wall + 2 dummy layers")
     out = {
-        "metric": "particle-steps/s at 3D 1M",
+        "metric": metric,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -225,9 +292,7 @@ def main():
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic",
         "config": {
-            "workload": "3D dam break, plain MPS, " + ("mixed FP32/FP64" if mixed else "FP64") +
-                        ", l0=0.005 m, water 100x100x80 in 1.6x0.6x0.405 m tank, "
-                        "1 wall + 2 dummy layers",
+            "workload": workload,
             "particles": case.tp,
             "fluid": case.fp,
             "re_s": "2.1 l0", "re_l": "4.0 l0", "dt": p.dt,
@@ -236,6 +301,8 @@ def main():
                    + (", warm start" if args.warm_start else ", cold start"),
             "cg_iters_mean": sum(iters) / len(iters),
             "parallelism": "1 GPU",
+            "search_walks": modes,
+            "solo_runs": solo,
         },
         "roofline": {
             "kernel": "k_cg_spmv (PPE q = A p + p.q)" + (", FP32" if mixed else ", FP64"),
@@ -244,7 +311,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic("mixed" if mixed else "fp64", nu, nnz),
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
             "launches": nl.value,

@@ -226,6 +226,21 @@ int vox_init(vox_ctx* ctx);
 int vox_step(vox_ctx* ctx, int nsteps);
 int vox_sync(vox_ctx* ctx);
 int vox_get_status(vox_ctx* ctx, vox_status* st);
+/* Batched scenarios (no counterpart in the reference; for ensembles of
+ * independent cases laid out apart from each other in one context): region[i]
+ * is particle i's scenario and boxes holds 6 bounds per scenario (xmin, xmax,
+ * ymin, ymax, zmin, zmax). BOUNDARIE (functions.cu:1963-2002) and the PREPDATA
+ * box check (functions.cu:2688-2746) then use the particle's own box instead of
+ * the grid box. nregion = 0 clears it. */
+int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const double* boxes);
+
+/* Tile-shape autotuner (no counterpart in the reference): times both
+ * neighbour-search walks (1: one thread per particle, 2: one wave per grid
+ * cell) over `reps` builds on the current positions and keeps the faster per
+ * list; modes[2] receives the choice for lists S and L. The lists themselves do
+ * not depend on it. Requires vox_init. */
+int vox_autotune(vox_ctx* ctx, int reps, int32_t* modes);
+
 /* Reference densities; vox_set_n0 lets a caller impose its own. */
 int vox_get_n0(vox_ctx* ctx, double* n0_s, double* n0_l);
 int vox_set_n0(vox_ctx* ctx, double n0_s, double n0_l);

@@ -98,10 +98,11 @@ def _blocked_order(ij, block):
     return np.lexsort(keys)
 
 
-def dam_break(dim, l0, water, tank_cells, wall_layers=1, dummy_layers=2, name=""):
+def dam_break(dim, l0, water, tank_cells, wall_layers=1, dummy_layers=2, name="", x_offset=0):
     """Dam break: a water block of `water` particles in the corner of an open
-    tank whose inner faces are at lattice 0 and `tank_cells` along each axis.
-    y is vertical (gravity along -y)."""
+    tank whose inner faces are at lattice 0 and `tank_cells` along each axis
+    (shifted by `x_offset` lattice cells along x). y is vertical (gravity along
+    -y)."""
     nl = wall_layers + dummy_layers
     if dim == 2:
         nx, ny = tank_cells
@@ -139,7 +140,7 @@ def dam_break(dim, l0, water, tank_cells, wall_layers=1, dummy_layers=2, name=""
     lat = np.concatenate([g[0] for g in groups], 0)
     ptype = np.concatenate([np.full(len(g[0]), g[1], np.int32) for g in groups])
     n = lat.shape[0]
-    x = lat[:, 0].astype(np.float64) * l0
+    x = (lat[:, 0] + x_offset).astype(np.float64) * l0
     y = lat[:, 1].astype(np.float64) * l0
     z = lat[:, 2].astype(np.float64) * l0 if dim == 3 else np.zeros(n)
     zeros = np.zeros(n)
@@ -162,6 +163,70 @@ def dam_break_3d_c1(scale=1):
     wx, wy, wz = 100 // scale, 100 // scale, 80 // scale
     return dam_break(3, 0.005, (wx, wy, wz), (320 // scale, 120 // scale, wz + 1), 1, 2,
                      name=f"dam3d_{wx}x{wy}x{wz}_l0.005")
+
+
+def two_phase(case, fraction=0.5):
+    """Fluid particles above `fraction` of the water column's height become
+    fluid 2 (PTYPE 2): a layered two-phase column."""
+    out = case.copy()
+    f = out.ptype > 0
+    y0, y1 = out.y[f].min(), out.y[f].max()
+    out.ptype[f & (out.y > y0 + fraction * (y1 - y0))] = 2
+    return out
+
+
+def box_of(case):
+    """Grid box of a case run on its own: bounding box +- l0 (main.cu:692-715)."""
+    l0 = case.l0
+    if case.dim == 3:
+        return (case.x.min() - l0, case.x.max() + l0, case.y.min() - l0, case.y.max() + l0,
+                case.z.min() - l0, case.z.max() + l0)
+    return (case.x.min() - l0, case.x.max() + l0, case.y.min() - l0, case.y.max() + l0, 0.0, 0.0)
+
+
+def ensemble_2d(widths, heights, l0=0.01, gap_cells=16):
+    """Independent 2D two-phase dam breaks laid out along x in one particle set.
+
+    Scenario k is a water column of widths[k] x heights[k] particles (lower
+    half fluid 1, upper half fluid 2) in an open tank 4 * width wide and
+    height + 20 high, with one wall and two dummy layers. Tanks are placed
+    `gap_cells` lattice cells apart (more than twice the largest neighbour
+    radius, so no particle of one scenario neighbours another). Particles are
+    ordered dummy -> wall -> fluid over all scenarios, each scenario keeping its
+    own relative order. Returns (case, region, boxes, scenarios): region[i] is
+    particle i's scenario, boxes the 6 box bounds of each scenario run alone,
+    and scenarios the cases each scenario forms alone (same coordinates)."""
+    scen = []
+    off = 0
+    for w, h in zip(widths, heights):
+        w, h = int(w), int(h)
+        tank = (4 * w, h + 20)
+        c = two_phase(dam_break(2, l0, (w, h), tank, 1, 2, name=f"col{w}x{h}", x_offset=off))
+        scen.append(c)
+        off += tank[0] + 2 + gap_cells
+    groups = []
+    region = []
+    for sel in (lambda t: t < 0, lambda t: t == 0, lambda t: t > 0):
+        for k, c in enumerate(scen):
+            m = sel(c.ptype)
+            groups.append((c, m))
+            region.append(np.full(int(m.sum()), k, np.int32))
+    cat = lambda attr: np.concatenate([getattr(c, attr)[m] for c, m in groups])
+    case = Case(2, l0, cat("x"), cat("y"), cat("z"), cat("u"), cat("v"), cat("w"), cat("p"),
+                cat("ptype").astype(np.int32), name=f"ensemble{len(scen)}",
+                meta={"widths": list(map(int, widths)), "heights": list(map(int, heights))})
+    boxes = np.array([box_of(c) for c in scen], np.float64)
+    return case, np.concatenate(region), boxes, scen
+
+
+def ensemble_c4(count=64, seed=0, l0=0.01):
+    """The batched ensemble: `count` two-phase 2D dam breaks with column
+    widths uniform in 20..200 and heights in 40..200 particles, drawn from
+    SplitMix64(seed) as (width, height) per scenario in order."""
+    r = uniform01(seed, 2 * count).reshape(count, 2)
+    widths = 20 + np.floor(r[:, 0] * 181).astype(np.int64)
+    heights = 40 + np.floor(r[:, 1] * 161).astype(np.int64)
+    return ensemble_2d(widths, heights, l0)
 
 
 def write_grd(path, case):

@@ -340,7 +340,7 @@ int vox_destroy(vox_ctx* c)
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
     dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
-    dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
+    dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->region); dfree(c->rbox); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
     dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
     if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->hscal) (void)hipHostFree(c->hscal);
@@ -555,6 +555,32 @@ int vox_get_n0(vox_ctx* c, double* n0s, double* n0l)
     if (n0s) *n0s = c->n0s;
     if (n0l) *n0l = c->n0l;
     return VOX_OK;
+}
+
+int vox_set_regions(vox_ctx* c, int nregion, const int32_t* region, const double* boxes)
+{
+    VOX_TRY(check_ctx(c));
+    if (c->region) (void)hipFree(c->region);
+    if (c->rbox) (void)hipFree(c->rbox);
+    c->region = nullptr;
+    c->rbox = nullptr;
+    c->nregion = 0;
+    if (nregion == 0) return VOX_OK;
+    if (nregion < 0 || !region || !boxes) return fail(VOX_ERR_ARG, "bad region arguments");
+    for (int i = 0; i < c->tp; ++i)
+        if (region[i] < 0 || region[i] >= nregion) return fail(VOX_ERR_ARG, "region id out of range");
+    VOX_HIP(hipMalloc(&c->region, sizeof(int32_t) * c->tp));
+    VOX_HIP(hipMalloc(&c->rbox, sizeof(double) * 6 * (size_t)nregion));
+    VOX_HIP(hipMemcpy(c->region, region, sizeof(int32_t) * c->tp, hipMemcpyHostToDevice));
+    VOX_HIP(hipMemcpy(c->rbox, boxes, sizeof(double) * 6 * (size_t)nregion, hipMemcpyHostToDevice));
+    c->nregion = nregion;
+    return VOX_OK;
+}
+
+int vox_autotune(vox_ctx* c, int reps, int32_t* modes)
+{
+    VOX_TRY(need_init(c));
+    return autotune_lists(c, reps, modes);
 }
 
 int vox_set_n0(vox_ctx* c, double n0s, double n0l)

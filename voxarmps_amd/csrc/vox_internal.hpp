@@ -187,6 +187,7 @@ struct vox_ctx {
     int32_t* bcon = nullptr;
 
     vox::DevList list[2];
+    int tuned_mode[2] = {0, 0};    // neighbour walk chosen by the autotuner (0: none)
     bool two_lists = false;
     double n0s = 0.0, n0l = 0.0;
     bool initialised = false;
@@ -233,6 +234,11 @@ struct vox_ctx {
     int32_t* coll_j = nullptr;
     int32_t* coll_stamp = nullptr; // tp
     int32_t* coll_root = nullptr;  // tp: smallest particle of each candidate component
+
+    // batched scenarios: region of each particle and 6 box bounds per region
+    int32_t* region = nullptr;
+    double* rbox = nullptr;
+    int nregion = 0;
     int32_t* coll_logidx = nullptr;// tp
     double* coll_log = nullptr;    // 8 doubles per logged particle
     int32_t* coll_logn = nullptr;  // 1
@@ -299,6 +305,7 @@ int fill_i32(vox_ctx* c, int32_t* p, int32_t v, long n);
 int fill_f64(vox_ctx* c, double* p, double v, long n);
 int read_flags(vox_ctx* c);   // D2H of dflags + dscal into pinned mirrors (syncs)
 
+int autotune_lists(vox_ctx* c, int reps, int32_t* modes);
 int build_neighbors(vox_ctx* c);
 int ensure_list_storage(vox_ctx* c, int which, int cap);
 

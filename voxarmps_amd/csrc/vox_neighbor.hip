@@ -474,8 +474,9 @@ static int build_one(vox_ctx* c, int which, double re)
     const double per_cell = (c->p.dim == 3)
         ? L.g.hx * L.g.hx * L.g.hz / (c->p.dl * c->p.dl * c->p.dl)
         : L.g.hx * L.g.hx / (c->p.dl * c->p.dl);
-    const bool cell_walk = (c->p.list_mode == 2) ||
-                           (c->p.list_mode == 0 && per_cell >= 24.0 && L.g.tnc * kCellHalves < (1L << 31));
+    const int mode = c->tuned_mode[which] ? c->tuned_mode[which] : c->p.list_mode;
+    const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0)) &&
+                           L.g.tnc * kCellHalves < (1L << 31);
     for (int attempt = 0; attempt < 4; ++attempt) {
         {
             ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
@@ -503,6 +504,45 @@ static int build_one(vox_ctx* c, int which, double re)
         VOX_TRY(ensure_list_storage(c, which, cap));
     }
     return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+// Tile-shape autotuner of the neighbour search: times both walks (one thread
+// per particle, one wave per cell) on the current positions, `reps` builds
+// each, and keeps the faster one per list. The lists are identical either way,
+// so the choice only changes speed. modes[w] receives 1 or 2 per list.
+int autotune_lists(vox_ctx* c, int reps, int32_t* modes)
+{
+    if (reps < 1) reps = 1;
+    hipEvent_t e0, e1;
+    VOX_HIP(hipEventCreate(&e0));
+    VOX_HIP(hipEventCreate(&e1));
+    int rc = VOX_OK;
+    for (int w = 0; w < (c->two_lists ? 2 : 1) && rc == VOX_OK; ++w) {
+        const double re = (w == 0) ? c->p.re_s : c->p.re_l;
+        float best = 0.f;
+        int pick = 1;
+        for (int mode = 1; mode <= 2 && rc == VOX_OK; ++mode) {
+            c->tuned_mode[w] = mode;
+            if ((rc = build_one(c, w, re)) != VOX_OK) break;       // warm (and grows rows)
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < reps && rc == VOX_OK; ++r) rc = build_one(c, w, re);
+            (void)hipEventRecord(e1, c->stream);
+            if (rc != VOX_OK) break;
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(VOX_ERR_HIP, "autotune sync"); break; }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (mode == 1 || ms < best) { best = ms; pick = mode; }
+        }
+        c->tuned_mode[w] = pick;
+        if (modes) modes[w] = pick;
+    }
+    if (!c->two_lists) {
+        c->tuned_mode[1] = c->tuned_mode[0];
+        if (modes) modes[1] = modes[0];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 int build_neighbors(vox_ctx* c)

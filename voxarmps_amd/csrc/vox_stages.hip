@@ -649,12 +649,29 @@ struct EulerArgs {
     int first, n, dim, kind;
     double dt, dl, cc, xmin, xmax, ymin, ymax, zmin, zmax;
     double *x, *y, *z, *u, *v, *w;
+    const int32_t* region;      // per-particle region (batched scenarios) or null
+    const double* rbox;         // 6 bounds per region
 };
+
+// The grid box bounds, or those of the particle's region when scenarios are
+// batched (each keeps the box of its own run).
+struct Box { double xmin, xmax, ymin, ymax, zmin, zmax; };
+__device__ __forceinline__ Box box_of(const int32_t* region, const double* rbox, int i, Box g)
+{
+    if (!region) return g;
+    const double* b = rbox + 6L * region[i];
+    return Box{b[0], b[1], b[2], b[3], b[4], b[5]};
+}
 
 __global__ void k_euler(EulerArgs a)
 {
     int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
+    {
+        const Box b = box_of(a.region, a.rbox, i, Box{a.xmin, a.xmax, a.ymin, a.ymax, a.zmin, a.zmax});
+        a.xmin = b.xmin; a.xmax = b.xmax; a.ymin = b.ymin; a.ymax = b.ymax; a.zmin = b.zmin;
+        a.zmax = b.zmax;
+    }
     double x = a.x[i] + a.u[i] * a.dt;
     double y = a.y[i] + a.v[i] * a.dt;
     double u = a.u[i], v = a.v[i];
@@ -689,6 +706,7 @@ int st_euler(vox_ctx* c)
     a.xmin = p.xmin; a.xmax = p.xmax; a.ymin = p.ymin; a.ymax = p.ymax; a.zmin = p.zmin; a.zmax = p.zmax;
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
     a.u = c->f[VOX_F_UNEW]; a.v = c->f[VOX_F_VNEW]; a.w = c->f[VOX_F_WNEW];
+    a.region = c->region; a.rbox = c->rbox;
     int m = c->tp - a.first;
     if (m <= 0) return VOX_OK;
     k_euler<<<grid_for(m), kBlock, 0, c->stream>>>(a);
@@ -1051,6 +1069,8 @@ struct PrepArgs {
     const double *x, *y, *z, *un, *vn, *wn, *pn;
     double *u, *v, *w, *p;
     int32_t* flags;
+    const int32_t* region;
+    const double* rbox;
 };
 
 __global__ __launch_bounds__(kBlock) void k_prepdata(PrepArgs a)
@@ -1070,14 +1090,15 @@ __global__ __launch_bounds__(kBlock) void k_prepdata(PrepArgs a)
             if (a.dim == 3) w = a.w[i];
         }
         double x = a.x[i], y = a.y[i];
+        const Box b = box_of(a.region, a.rbox, i, Box{a.xmin, a.xmax, a.ymin, a.ymax, a.zmin, a.zmax});
         if (a.dim == 3) {
             double z = a.z[i];
             if (pv * 0 != 0 || u * 0 != 0 || v * 0 != 0 || w * 0 != 0) nanc = 1;
-            if (x >= a.xmax || y >= a.ymax || z >= a.zmax || x <= a.xmin || y <= a.ymin || z <= a.zmin)
+            if (x >= b.xmax || y >= b.ymax || z >= b.zmax || x <= b.xmin || y <= b.ymin || z <= b.zmin)
                 oob = 1;
         } else {
             if (pv * 0 != 0 || u * 0 != 0 || v * 0 != 0) nanc = 1;
-            if (x >= a.xmax || y >= a.ymax || x <= a.xmin || y <= a.ymin) oob = 1;
+            if (x >= b.xmax || y >= b.ymax || x <= b.xmin || y <= b.ymin) oob = 1;
         }
     }
     // one atomic per wave
@@ -1100,6 +1121,7 @@ int st_prepdata(vox_ctx* c)
     a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW]; a.pn = c->f[VOX_F_PNEW];
     a.u = c->f[VOX_F_U]; a.v = c->f[VOX_F_V]; a.w = c->f[VOX_F_W]; a.p = c->f[VOX_F_P];
     a.flags = c->dflags;
+    a.region = c->region; a.rbox = c->rbox;
     VOX_TRY(fill_i32(c, c->dflags + FLAG_NAN, 0, 2));
     k_prepdata<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());

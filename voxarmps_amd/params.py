@@ -98,3 +98,30 @@ def dam3d_params(case, dt=5e-5, tol=1e-8):
     p.dt = dt
     p.cg_tol = tol
     return set_box(p, case)
+
+
+def ensemble_params(case, dt=1e-4, tol=1e-8, box=None):
+    """The batched two-phase ensemble (also each scenario run alone, with
+    `box` = its own box): 2D, re 2.1 l0 / 4.0 l0, dt 1e-4 s; fluid 1 water
+    (rho 1000, nu 1e-6), fluid 2 rho 850, nu 1e-3 with a Bingham yield stress
+    of 1 Pa (Herschel-Bulkley with N = 1, MEU0 = rho2 nu2, cohesion 1 Pa, phi
+    0); SPS-LES on with Cs 0.12; mixed precision, PPE to rel-res `tol`."""
+    from ._abi import PREC_MIXED
+    p = default_params(2, case.l0)
+    two_radius(p)
+    p.gy = -9.81
+    p.dt = dt
+    p.cg_tol = tol
+    p.rho2, p.nu2 = 850.0, 1e-3
+    p.fluid2_type = 1
+    p.cohes = 1.0
+    p.phi = 0.0
+    p.n_pl = 1.0
+    p.meu0 = p.rho2 * p.nu2
+    p.turb = 1
+    p.cs = 0.12
+    p.precision = PREC_MIXED
+    if box is None:
+        return set_box(p, case)
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = (float(v) for v in box)
+    return p

@@ -221,6 +221,22 @@ class Simulation:
         self.be.check(self.be.fn("get_cells")(self.ctx, which, _ptr(a, C.c_int32)), "get_cells")
         return a
 
+    def set_regions(self, region, boxes):
+        """Batched scenarios: region[i] is particle i's scenario, boxes[k] the
+        6 box bounds scenario k uses for BOUNDARIE and the PREPDATA check."""
+        region = np.ascontiguousarray(region, np.int32)
+        boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 6)
+        self._keep = (region, boxes)
+        self.be.check(self.be.fn("set_regions")(self.ctx, boxes.shape[0], _ptr(region, C.c_int32),
+                                                _ptr(boxes, C.c_double)), "set_regions")
+
+    def autotune(self, reps=3):
+        """Times both neighbour-search walks and keeps the faster per list;
+        returns the chosen walk (1 per particle, 2 per cell) for lists S, L."""
+        m = np.zeros(2, np.int32)
+        self.be.check(self.be.fn("autotune")(self.ctx, int(reps), _ptr(m, C.c_int32)), "autotune")
+        return int(m[0]), int(m[1])
+
     def ppe(self):
         """Assembled PPE as CSR over the bcon == 0 rows: dict with rowptr, cols,
         vals (diagonal first in each row), rhs and the particle id of each row."""
