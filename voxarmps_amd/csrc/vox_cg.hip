@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
 
 // Row products over the SELL-64 matrix, entries in row order after the
 // diagonal: acc = d_r p_r + sum_k a_rk p_col(rk). A wave's entry k of its 64
-// rows is one contiguous block; 16 entries' loads are in flight per lane. The
+// rows is one contiguous block; 8 entries' loads are in flight per lane. The
 // matrix streams from HBM once per product (non-temporal loads keep it from
 // evicting the gathered vector from L2).
 
@@ -126,15 +126,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
     double pq = 0.0;
     const int r = xcd_tid();
     if (r < a.nu) {
-        // (all 32 streaming loads of a step are issued before the first
+        // (all 16 streaming loads of a step are issued before the first
         // gather: keep this loop's shape, other forms scheduled the gathers
-        // earlier and measured about 18% slower)
+        // earlier and measured about 18% slower; 8 entries per step measured
+        // 11% faster than 16 and 4)
         const VT pr = a.p[r];
         VT acc = (VT)a.diag[r] * pr;
         const int len = a.rowlen[r];
         const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
         const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
-        constexpr int U = 16;
+        constexpr int U = 8;
         int k = 0;
         for (; k + U <= len; k += U) {
             int32_t cc[U];
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
         const int len = a.rowlen[r];
         const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
         const double* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
-        constexpr int U = 16;
+        constexpr int U = 8;
         int k = 0;
         for (; k + U <= len; k += U) {
             int32_t cc[U];

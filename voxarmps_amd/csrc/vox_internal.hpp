@@ -365,6 +365,36 @@ __device__ __forceinline__ double wfun(double r, int ktype, double re)
     return 0.0;
 }
 
+// a / b, correctly rounded, from y = 1.0 / b correctly rounded: q = a*y is
+// within one ulp of a/b, the residual a - b*q is exact with an FMA, and one
+// correction step q + (a - b*q)*y rounds to exactly the IEEE quotient
+// (Markstein's theorem; binary64, no overflow or underflow on the way). It
+// replaces a full division when several quotients share one divisor. b = 0
+// gives y = inf and a non-finite result, as a / 0 does up to the sign of NaN;
+// callers mask those entries.
+__device__ __forceinline__ double div_by(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = fma(-q, b, a);
+    return fma(r, y, q);
+}
+
+// wfun with the division by re done through inv_re = 1.0 / re (identical
+// result, see div_by).
+__device__ __forceinline__ double wfun_r(double r, int ktype, double re, double inv_re)
+{
+    double q = div_by(r, re, inv_re);
+    if (ktype == 2) return (q < 1.0) ? (1 / q) - 1 : 0.0;
+    if (ktype == 6) return (q <= 1.0) ? (1 - q) * (1 - q) * (1 - q) : 0.0;
+    if (ktype == 1) {
+        if (q < 0.5) return 2.0 - 4.0 * (q * q);
+        if (q <= 1.0) return (2 * q - 2) * (2 * q - 2);
+        return 0.0;
+    }
+    if (ktype == 5) return (q <= 1.0) ? 1.5 * log(1 / (q + 0.000001)) : 0.0;
+    return 0.0;
+}
+
 // Fixed-order block sum of one double per thread (blockDim == kBlock).
 __device__ __forceinline__ double block_sum(double v, double* lds)
 {

@@ -36,6 +36,7 @@ __global__ void k_umap(int n, const int32_t* __restrict__ bcon, int32_t* __restr
 struct AsmArgs {
     int n, dim, ktype, acap;
     double re, c0, n0l, rho1, dshift, inv_dt_dt, n0s;
+    double inv_re, inv_n0l, inv_rho1;     // correctly rounded reciprocals
     ListView L;
     const double *x, *y, *z, *pnd;
     const int32_t *bcon, *uidx;
@@ -98,7 +99,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             const double d = sqrt(dx * dx + dy * dy + dz * dz);
             // jj == i: self, or padding past the row end
             const bool use = (jj[u] != i) && (bj != -1) && !(fabs(d) <= 1.0e-15);
-            const double val = a.c0 * wfun(d, a.ktype, a.re) / a.n0l / a.rho1;
+            // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by)
+            const double val = div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
+                                      a.rho1, a.inv_rho1);
             if (use && bj == 0 && val != 0.0) {
                 a.acol[ob + 64L * k] = rj.aux;
                 a.aval[ob + 64L * k] = -val;
@@ -162,6 +165,7 @@ int ppe_assemble(vox_ctx* c)
     a.n = n; a.dim = p.dim; a.ktype = p.ktype; a.re = p.re_l;
     a.c0 = 2.0 * p.dim / p.lambda;
     a.n0l = c->n0l; a.rho1 = p.rho1;
+    a.inv_re = 1.0 / a.re; a.inv_n0l = 1.0 / a.n0l; a.inv_rho1 = 1.0 / a.rho1;
     a.dshift = (1.00e-7) / p.dt / p.dt;
     a.inv_dt_dt = 1.0 / p.dt / p.dt;
     a.n0s = c->n0s;

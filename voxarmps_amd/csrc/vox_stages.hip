@@ -53,6 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, doubl
     if (i >= n) return;
     double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
     double sum = 0.0;
+    const double inv_re = 1.0 / re;
     int cnt = L.cnt[i];
     const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
     for (int g = 0; (g << 3) < cnt; ++g) {
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, doubl
             int j = jc[e];
             if (j == i) continue;
             double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
-            sum = sum + wfun(d, ktype, re);
+            sum = sum + wfun_r(d, ktype, re, inv_re);
         }
     }
     if (ktype != 2) sum = sum * ncorr;
@@ -406,6 +407,8 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
     const double xi = me.x, yi = me.y, zi = me.z, ui = me.u, vi = me.v, wi = me.w;
     const double mi = me.meu, phi_ = a.ph[i];
     double rhoN = (ti == 1) ? a.rho1 : a.rho2;
+    const double inv_rhoN = 1.0 / rhoN;
+    const double inv_re_s = 1.0 / a.re_s, inv_re_l = 1.0 / a.re_l;
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
     const int cnt = a.L.cnt[i];
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
@@ -436,19 +439,22 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
         const bool use = (jj[u] != i) && (D > 0);
         const double mj = rj.meu;
-        const double hm = 2 * mi * mj / (mi + mj) / rhoN;
-        const double NEU = (ti == tj[u]) ? mi / rhoN : hm;
-        const double wg = wfun(D, a.ktype, a.re_s);
-        const double wl = (a.re_l == a.re_s) ? wg : wfun(D, a.ktype, a.re_l);
+        // every quotient below equals the plain IEEE division (div_by); the
+        // six divisions by D share one reciprocal
+        const double inv_D = 1.0 / D;
+        const double hm = div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
+        const double NEU = (ti == tj[u]) ? div_by(mi, rhoN, inv_rhoN) : hm;
+        const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
+        const double wl = (a.re_l == a.re_s) ? wg : wfun_r(D, a.ktype, a.re_l, inv_re_l);
         const double dp = rj.pn - phi_;
         const double du = rj.u - ui, dv = rj.v - vi, dw = rj.w - wi;
-        s1 = use ? s1 + dp * dx * wg / D / D : s1;
-        s2 = use ? s2 + dp * dy * wg / D / D : s2;
+        s1 = use ? s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D) : s1;
+        s2 = use ? s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D) : s2;
         s4 = use ? s4 + wl * du * NEU : s4;
         s5 = use ? s5 + wl * dv * NEU : s5;
         s7 = use ? s7 + wl * du : s7;
         s8 = use ? s8 + wl * dv : s8;
-        s3 = use ? s3 + dp * dz * wg / D / D : s3;   // z sums: only read in 3D
+        s3 = use ? s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D) : s3;   // z sums: only read in 3D
         s6 = use ? s6 + wl * dw * NEU : s6;
         s9 = use ? s9 + wl * dw : s9;
     }
@@ -587,6 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
     double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
     double pi_ = a.pn[i], phi_ = a.ph[i];
     double s1 = 0, s2 = 0, s3 = 0;
+    const double inv_re = 1.0 / a.re;
     int cnt = a.L.cnt[i];
     const int32_t* __restrict__ rowp = a.L.ids + sl_index(i, 0, a.L.cap);
     for (int g = 0; (g << 3) < cnt; ++g) {
@@ -600,11 +607,12 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
             double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
             if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
             else D = sqrt(dx * dx + dy * dy);
-            double wv = wfun(D, a.ktype, a.re);
+            double wv = wfun_r(D, a.ktype, a.re, inv_re);
             double dp = (a.kh == 1) ? (a.pn[j] + pi_ - (phi_ + a.ph[j])) : (a.pn[j] - phi_);
-            s1 = s1 + dp * dx * wv / D / D;
-            s2 = s2 + dp * dy * wv / D / D;
-            if (dim == 3) s3 = s3 + dp * dz * wv / D / D;
+            const double inv_D = 1.0 / D;                 // x / D / D via div_by
+            s1 = s1 + div_by(div_by(dp * dx * wv, D, inv_D), D, inv_D);
+            s2 = s2 + div_by(div_by(dp * dy * wv, D, inv_D), D, inv_D);
+            if (dim == 3) s3 = s3 + div_by(div_by(dp * dz * wv, D, inv_D), D, inv_D);
         }
     }
     double g = a.relaxp * (a.gnum * a.dt / a.n0 / a.rho[i]);
