@@ -40,8 +40,8 @@ struct AsmArgs {
     ListView L;
     const double *x, *y, *z, *pnd;
     const int32_t *bcon, *uidx;
-    int32_t *acol, *rowlen;
-    double *aval, *adiag, *rhs, *source;
+    int32_t *acol, *rowlen, *tcol;         // tcol/tval: chunked staging rows
+    double *aval, *adiag, *rhs, *source, *tval;
     PosRec* rec;
 };
 
@@ -57,8 +57,32 @@ __global__ void k_pack_asm(AsmArgs a)
     a.rec[i] = r;
 }
 
+// Per-thread ring of the last 16 outputs in LDS (strides of 17 keep the lanes'
+// equal-slot accesses on distinct banks).
+constexpr int kRing = 17;
+
+// Stores chunk c (8 entries) of the ring to the chunked staging rows: one
+// 32-byte and one 64-byte contiguous store per lane.
+__device__ __forceinline__ void flush_chunk(const int32_t* rc, const double* rv, int c,
+                                            int32_t* __restrict__ tcol, double* __restrict__ tval,
+                                            long rb)
+{
+    const int s0 = (c & 1) * 8;
+    const long o = rb + ((long)c << 9);
+    v4i* c4 = reinterpret_cast<v4i*>(tcol + o);
+    c4[0] = v4i{rc[s0 + 0], rc[s0 + 1], rc[s0 + 2], rc[s0 + 3]};
+    c4[1] = v4i{rc[s0 + 4], rc[s0 + 5], rc[s0 + 6], rc[s0 + 7]};
+    v2d* v2 = reinterpret_cast<v2d*>(tval + o);
+    v2[0] = v2d{rv[s0 + 0], rv[s0 + 1]};
+    v2[1] = v2d{rv[s0 + 2], rv[s0 + 3]};
+    v2[2] = v2d{rv[s0 + 4], rv[s0 + 5]};
+    v2[3] = v2d{rv[s0 + 6], rv[s0 + 7]};
+}
+
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 {
+    __shared__ int32_t ring_c[kBlock * kRing];
+    __shared__ double ring_v[kBlock * kRing];
     int i = xcd_tid();
     if (i >= a.n) return;
     const int bi = a.bcon[i];
@@ -68,7 +92,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     a.source[i] = src;
     if (bi != 0) return;
     const int r = a.uidx[i];
-    const long ob = mx_index(r, 0, a.acap);
+    const long rb = sl_index(r, 0, a.acap);               // chunked staging rows
+    int32_t* rc = ring_c + threadIdx.x * kRing;
+    double* rv = ring_v + threadIdx.x * kRing;
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
     const PosRec me = a.rec[i];
     const double xi = me.x, yi = me.y, zi = me.z;
@@ -76,8 +102,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
     int k = 0;
     const int cnt = a.L.cnt[i];
     // One row chunk of 8 per step: all index and record loads of a chunk are
-    // issued before its stores (the stores would otherwise order every next
-    // load behind them); entries are then processed strictly in row order.
+    // issued before its stores; entries are processed strictly in row order.
+    // Kept entries go to the thread's LDS ring; each completed group of 8 is
+    // stored at once (whole sectors), so no partly written line is left to be
+    // written back again and again.
     constexpr int U = 8;
     for (int l0 = 0; l0 < cnt; l0 += U) {
         int jj[U];
@@ -87,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
         for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
         for (int u = 0; u < U; ++u) rr[u] = a.rec[jj[u]];
+        const int k0 = k;
         // Branch-free per entry (a skipped entry keeps diag unchanged through a
         // select), so no record load is sunk into a branch behind a full wait.
 #pragma unroll
@@ -103,17 +132,52 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             const double val = div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
                                       a.rho1, a.inv_rho1);
             if (use && bj == 0 && val != 0.0) {
-                a.acol[ob + 64L * k] = rj.aux;
-                a.aval[ob + 64L * k] = -val;
+                rc[k & 15] = rj.aux;
+                rv[k & 15] = -val;
                 ++k;
             }
             diag = use ? diag + val : diag;
         }
+        // at most 8 outputs per step: at most one group of 8 completed
+        if ((k0 >> 3) != (k >> 3)) flush_chunk(rc, rv, k0 >> 3, a.tcol, a.tval, rb);
     }
+    if (k & 7) flush_chunk(rc, rv, k >> 3, a.tcol, a.tval, rb);   // the last, partial group
     diag += a.dshift;
     a.adiag[r] = diag;
     a.rowlen[r] = k;
     a.rhs[r] = src;
+}
+
+// The packed rows from the chunked staging layout into SELL-64: a slice's
+// group g of 64 rows is one contiguous block there (lane t's 8 entries at
+// offset 8t), and entry 8g + u of the 64 rows is one contiguous block here, so
+// every load and store of a wave is contiguous. One thread per row.
+__global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
+                                                          const int32_t* __restrict__ rowlen,
+                                                          const int32_t* __restrict__ tcol,
+                                                          const double* __restrict__ tval,
+                                                          int32_t* __restrict__ acol,
+                                                          double* __restrict__ aval)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nu) return;
+    const long rb = sl_index(r, 0, cap);
+    const long ob = mx_index(r, 0, cap);
+    const int len = rowlen[r];
+    for (int g = 0; (g << 3) < len; ++g) {
+        int32_t cc[8];
+        double vv[8];
+        load8_nt(tcol + rb + ((long)g << 9), cc);
+        load8_nt(tval + rb + ((long)g << 9), vv);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = (g << 3) + u;
+            if (kk < len) {
+                acol[ob + 64L * kk] = cc[u];
+                aval[ob + 64L * kk] = vv[u];
+            }
+        }
+    }
 }
 
 // FP32 copy of the off-diagonal values for the mixed-precision inner solves:
@@ -144,12 +208,12 @@ int ppe_assemble(vox_ctx* c)
     const int n = c->tp;
     const DevList& L = list_of(c, VOX_LIST_L);
     if (c->a_cap < L.cap) {
-        if (c->acol) (void)hipFree(c->acol);
-        if (c->aval) (void)hipFree(c->aval);
-        c->acol = nullptr;
-        c->aval = nullptr;
+        for (int32_t** q : {&c->acol, &c->tcol}) { if (*q) (void)hipFree(*q); *q = nullptr; }
+        for (double** q : {&c->aval, &c->tval}) { if (*q) (void)hipFree(*q); *q = nullptr; }
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(n, L.cap)));
+        VOX_HIP(hipMalloc(&c->tcol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
+        VOX_HIP(hipMalloc(&c->tval, sizeof(double) * (size_t)sl_size(n, L.cap)));
         if (c->aval32) (void)hipFree(c->aval32);
         c->aval32 = nullptr;
         c->a_cap = L.cap;
@@ -174,6 +238,7 @@ int ppe_assemble(vox_ctx* c)
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z]; a.pnd = c->f[VOX_F_N];
     a.bcon = c->bcon; a.uidx = c->uidx;
     a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
+    a.tcol = c->tcol; a.tval = c->tval;
     a.adiag = c->adiag; a.rhs = c->rhs;
     a.source = c->f[VOX_F_SOURCE];
     a.rec = c->pos_rec;
@@ -182,6 +247,12 @@ int ppe_assemble(vox_ctx* c)
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
+    if (c->nu > 0) {
+        k_asm_transpose<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen,
+                                                                   c->tcol, c->tval, c->acol,
+                                                                   c->aval);
+        VOX_HIP(hipGetLastError());
+    }
     if (mixed && c->nu > 0) {
         k_aval32<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen, c->aval,
                                                            c->aval32);
