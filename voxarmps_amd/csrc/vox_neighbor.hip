@@ -257,6 +257,23 @@ __device__ __forceinline__ int wave_max_i32(int v)
     return v;
 }
 
+// Row output through a per-thread ring of 16 ids in LDS: a completed group of
+// 8 entries is stored at once (one 32-byte store into the lane's own chunk),
+// instead of 8 separate 4-byte stores that would leave the sector partly
+// written several times. Groups past the row capacity are not stored (the
+// overflow is reported and the search repeats with wider rows).
+constexpr int kIdRing = 17;   // stride: lanes' equal slots fall on distinct banks
+
+__device__ __forceinline__ void flush_ids(const int32_t* ring, int c, int cap,
+                                          int32_t* __restrict__ row)
+{
+    if ((c << 3) >= cap) return;
+    const int s0 = (c & 1) * 8;
+    v4i* o = reinterpret_cast<v4i*>(row + ((long)c << 9));
+    o[0] = v4i{ring[s0 + 0], ring[s0 + 1], ring[s0 + 2], ring[s0 + 3]};
+    o[1] = v4i{ring[s0 + 4], ring[s0 + 5], ring[s0 + 6], ring[s0 + 7]};
+}
+
 __global__ __launch_bounds__(kBlock) void k_build_list(
     Grid g, int n, double R, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const int32_t* __restrict__ cell,
@@ -264,6 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
     int flag_overflow, int flag_max, int32_t* __restrict__ blockmax)
 {
+    __shared__ int32_t ring_all[kBlock * kIdRing];
+    int32_t* ring = ring_all + threadIdx.x * kIdRing;
     const int i = xcd_tid();
     int k = 0;
     if (i < n) {
@@ -292,6 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                         PosRec cr[U];
 #pragma unroll
                         for (int u = 0; u < U; ++u) cr[u] = rec[(j0 + u < e) ? j0 + u : s];
+                        const int k0 = k;
 #pragma unroll
                         for (int u = 0; u < U; ++u) {
                             // branch-free test (the loads above stay unconditional)
@@ -304,12 +324,15 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                                 acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
                             acc = acc && (j0 + u < e) && (g.dim == 3 || cu.id != i);
                             if (acc) {
-                                if (k < cap) ids[rb + sl_step(k)] = cu.id;
+                                ring[k & 15] = cu.id;
                                 ++k;
                             }
                         }
+                        // at most 8 accepts per chunk: at most one group completed
+                        if ((k0 >> 3) != (k >> 3)) flush_ids(ring, k0 >> 3, cap, ids + rb);
                     }
                 }
+        if (k & 7) flush_ids(ring, k >> 3, cap, ids + rb);
         cnt[i] = k;
         if (k > cap) atomicAdd(&flags[flag_overflow], 1);
     }
@@ -342,6 +365,8 @@ __global__ __launch_bounds__(kWave) void k_build_list_cells(
     int flag_overflow, int flag_max)
 {
     __shared__ PosRec tile[kWave];
+    __shared__ int32_t ring_all[kWave * kIdRing];
+    int32_t* ring = ring_all + threadIdx.x * kIdRing;
     const int b = xcd_block();
     const long cellid = b / kCellHalves;
     const int half = b % kCellHalves;
@@ -376,23 +401,31 @@ __global__ __launch_bounds__(kWave) void k_build_list_cells(
                         __syncthreads();                   // last chunk's reads are done
                         if (lane < nc) tile[lane] = rec[j0 + lane];
                         __syncthreads();
-#pragma unroll 4
-                        for (int u = 0; u < nc; ++u) {
-                            const PosRec cu = tile[u];
-                            // z is 0 in 2D records: dx^2+dy^2+0 equals the 2D sum
-                            const double dx = cu.x - me.x, dy = cu.y - me.y, dz = cu.z - me.z;
-                            const double d2 = dx * dx + dy * dy + dz * dz;
-                            bool acc = d2 < lo;
-                            if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
-                                acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
-                            acc = acc && active && (g.dim == 3 || cu.id != i);
-                            if (acc) {
-                                if (k < cap) ids[rb + sl_step(k)] = cu.id;
-                                ++k;
+                        // groups of 8 candidates: at most one group of 8
+                        // accepts completes per group
+                        for (int u0 = 0; u0 < nc; u0 += 8) {
+                            const int k0 = k;
+#pragma unroll
+                            for (int uu = 0; uu < 8; ++uu) {
+                                const int u = u0 + uu;
+                                const PosRec cu = tile[u < nc ? u : 0];
+                                // z is 0 in 2D records: dx^2+dy^2+0 equals the 2D sum
+                                const double dx = cu.x - me.x, dy = cu.y - me.y, dz = cu.z - me.z;
+                                const double d2 = dx * dx + dy * dy + dz * dz;
+                                bool acc = d2 < lo;
+                                if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
+                                    acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                                acc = acc && active && (u < nc) && (g.dim == 3 || cu.id != i);
+                                if (acc) {
+                                    ring[k & 15] = cu.id;
+                                    ++k;
+                                }
                             }
+                            if ((k0 >> 3) != (k >> 3)) flush_ids(ring, k0 >> 3, cap, ids + rb);
                         }
                     }
                 }
+        if (active && (k & 7)) flush_ids(ring, k >> 3, cap, ids + rb);
         if (active) {
             cnt[i] = k;
             if (k > cap) atomicAdd(&flags[flag_overflow], 1);
