@@ -12,6 +12,13 @@ resident in HBM.
 Prints one JSON line. The roofline entry is for the PPE SpMV kernel (the CG
 K1 pass): algorithmic bytes per launch / mean launch time from HIP events
 recorded on the library's stream during the timed steps.
+
+`--solver learned` runs the step loop with flexible CG and the learned
+preconditioner (weights from voxarmps_amd/data/lpc_weights.npy, written by
+python -m voxarmps_amd.train_lpc). With the weights present, the line also
+carries "ppe_solve": the last step's assembled system solved cold to the same
+relative residual by Jacobi-CG and by the learned flexible CG (network
+included), wall time between device synchronisations, median of 3.
 """
 import argparse
 import ctypes as C
@@ -144,6 +151,46 @@ def precision_report(checkpoints, scale=1):
     b.close()
 
 
+def ppe_compare(sim, p, net, reps=3):
+    """The assembled system of the last step solved cold by Jacobi-CG and by
+    the learned flexible CG: median wall ms between synchronisations,
+    iterations and relative residuals."""
+    import voxarmps_amd as vx
+    be = sim.be
+    out = {}
+    for name, solver in (("jacobi_cg", vx._abi.SOLVER_JACOBI_CG),
+                         ("learned_fcg", vx._abi.SOLVER_LEARNED_FCG)):
+        q = p.copy()
+        q.solver = solver
+        sim.set_params(q)
+        sim.attach_lpc(net)          # resets the baseline: the next learned solve fixes it
+        sim.CGM()
+        times, info = [], None
+        be.lib.vox_prof_reset(sim.ctx)
+        be.lib.vox_prof_enable(sim.ctx, 1)
+        for _ in range(reps):
+            be.lib.vox_sync(sim.ctx)
+            t0 = time.perf_counter()
+            info = sim.CGM()
+            be.lib.vox_sync(sim.ctx)
+            times.append((time.perf_counter() - t0) * 1e3)
+        be.lib.vox_prof_enable(sim.ctx, 0)
+        prof = {}
+        for k in ("lpc_net", "lpc_apply", "spmv"):
+            t = C.c_double(0.0)
+            n = C.c_int64(0)
+            be.lib.vox_prof_get(sim.ctx, k.encode(), C.byref(t), C.byref(n))
+            prof[k + "_ms"] = t.value / reps
+        times.sort()
+        out[name] = {"ms": times[len(times) // 2], "iterations": info.iterations,
+                     "relres": info.relres, "converged": bool(info.converged),
+                     "fallbacks": sim.status().lpc_fallbacks, **prof}
+    sim.set_params(p)
+    out["speedup"] = out["jacobi_cg"]["ms"] / out["learned_fcg"]["ms"]
+    out["system"] = "last step's PPE, cold start, rel-res %g" % p.cg_tol
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +209,10 @@ def main():
                     help="fp64: every kernel in FP64 (the reference's arithmetic); mixed: FP32 "
                          "velocity stages and FP32 inner CG inside FP64 iterative refinement "
                          "(default: fp64 for dam3d, mixed for ensemble)")
+    ap.add_argument("--solver", choices=("jacobi", "learned"), default="jacobi",
+                    help="PPE solver of the step loop: Jacobi-PCG, or flexible CG with the "
+                         "learned preconditioner (Jacobi-CG fallback)")
+    ap.add_argument("--lpc-weights", default=None, help="weights of the learned preconditioner")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
     ap.add_argument("--precision-report", type=str, default="", metavar="STEPS",
@@ -202,7 +253,17 @@ def main():
     mixed = precision == "mixed"
     p.precision = vx._abi.PREC_MIXED if mixed else vx._abi.PREC_FP64
     p.warm_start = 1 if args.warm_start else 0
+    net = None
+    wpath = args.lpc_weights or os.path.join(ROOT, "voxarmps_amd", "data", "lpc_weights.npy")
+    if os.path.exists(wpath) or args.solver == "learned":
+        from voxarmps_amd.train_lpc import load_weights
+        net = vx.LearnedPreconditioner(backend=be)
+        net.set_weights(load_weights(wpath))
+    if args.solver == "learned":
+        p.solver = vx._abi.SOLVER_LEARNED_FCG
     sim = vx.Simulation(p, case, backend=be)
+    if net is not None:
+        sim.attach_lpc(net)
     modes = None
     if ensemble:
         sim.set_regions(region, boxes)
@@ -239,12 +300,14 @@ def main():
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     per_kernel = {}
     for k in ("list", "list_l", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p",
-              "ir_residual",
+              "ir_residual", "lpc_net", "lpc_apply",
               "collision", "pgrad", "phat", "prep", "euler", "bcon", "visc", "vfrac", "misc"):
         t = C.c_double(0.0)
         n = C.c_int64(0)
         be.lib.vox_prof_get(sim.ctx, k.encode(), C.byref(t), C.byref(n))
         per_kernel[k] = round(t.value / max(args.steps, 1), 4)
+    ppe = ppe_compare(sim, p, net) if (net is not None and not ensemble) else None
+    fallbacks = st.lpc_fallbacks
     sim.close()
 
     solo = None
@@ -300,6 +363,8 @@ def main():
                     f"rel-res {p.cg_tol:g}" if mixed else f"Jacobi-PCG FP64 to rel-res {p.cg_tol:g}")
                    + (", warm start" if args.warm_start else ", cold start"),
             "cg_iters_mean": sum(iters) / len(iters),
+            "solver": ("flexible CG, learned preconditioner (Jacobi-CG fallback: "
+                       f"{fallbacks} fallbacks)") if args.solver == "learned" else "Jacobi-PCG",
             "parallelism": "1 GPU",
             "search_walks": modes,
             "solo_runs": solo,
@@ -319,6 +384,7 @@ def main():
             "nnz": nnz,
         },
         "stage_ms_per_step": per_kernel,
+        "ppe_solve": ppe,
         "cpu_baseline": cpu,
         "status": {"collisions": st.collisions, "nan": st.nan_count, "out_of_box": st.out_of_box,
                    "max_count_s": st.max_count_s, "max_count_l": st.max_count_l},

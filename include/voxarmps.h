@@ -62,7 +62,9 @@ enum vox_field {
 #define VOX_PREC_MIXED 1   /* FP32 particle kernels + SpMV, FP64 dots + IR   */
 
 /* PPE solvers */
-#define VOX_SOLVER_JACOBI_CG 0
+#define VOX_SOLVER_JACOBI_CG   0
+#define VOX_SOLVER_LEARNED_FCG 1  /* flexible CG with the learned preconditioner of an
+                                     attached vox_lpc; Jacobi-CG is the fallback */
 
 /* boundary-clamp variants of BOUNDARIE (functions.cu:1963-2002) */
 #define VOX_TEST_DAM  0
@@ -119,7 +121,8 @@ typedef struct vox_params {
                              0: from zero, as the reference (functions.cu:2503) */
     int32_t list_mode;    /* neighbour search: 0 auto, 1 one thread per particle,
                              2 one wave per grid cell (same lists either way)    */
-    int32_t reserved1;
+    int32_t lpc_cap_pct;  /* learned FCG: iteration cap in percent of the Jacobi-CG
+                             baseline before the fallback (200: twice)         */
 } vox_params;
 
 /* PPE solve report */
@@ -145,6 +148,8 @@ typedef struct vox_status {
     double  last_relres;      /* residual of the last solve                      */
     double  n0_s, n0_l;       /* reference densities                             */
     double  time;             /* simulated time                                  */
+    int32_t lpc_fallbacks;    /* learned solves that fell back to Jacobi-CG      */
+    int32_t lpc_baseline;     /* Jacobi-CG iterations the learned solver is held to */
 } vox_status;
 
 typedef struct vox_ctx vox_ctx;
@@ -261,12 +266,56 @@ int vox_get_cells(vox_ctx* ctx, int which, int32_t* cell);
 int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
                    int32_t* cols, double* vals, double* rhs, int32_t* ids);
 
+/* ---- learned preconditioner (no counterpart in the reference) ------------ */
+/* A message-passing network over the PPE matrix graph (hidden width 64,
+ * messages of 16, 3 layers, bf16 weights on the matrix cores with FP32
+ * accumulation) maps per-unknown features (PND ratio, wall flag, off-diagonal
+ * dominance, first moments of the neighbour offsets) to a diagonal scaling g
+ * and a coupling factor c of the preconditioner
+ *   z_r = g_r w_r / A_rr + sqrt(c_r) sum_{k strong} O_rk / (A_rr A_kk) sqrt(c_k) w_k
+ * (O = -offdiag(A); strong: O_rk at least a quarter of the coupling of two
+ * particles one spacing apart, about the neighbours within 2.3 spacings),
+ * used inside flexible CG. Training runs on the GPU over PPE systems dumped from
+ * contexts, minimising the mean of ||A M w - w||^2 / ||w||^2 over probes w
+ * (the right-hand side and Rademacher vectors) with Adam.
+ * Parameters are VOX_LPC_NPARAM floats in the packed order W0[64][8], b0[64],
+ * then per layer l = 1..3: P[16][64], U[64][64], V[64][16], c[64], then
+ * Wo[2][64], bo[2]. */
+#define VOX_LPC_NPARAM 19330
+typedef struct vox_lpc vox_lpc;
+/* New network on the current device; weights from SplitMix64(seed). */
+int vox_lpc_create(vox_lpc** out, uint64_t seed);
+int vox_lpc_destroy(vox_lpc* lpc);
+int vox_lpc_get_weights(vox_lpc* lpc, float* w);
+int vox_lpc_set_weights(vox_lpc* lpc, const float* w);
+/* Append the context's assembled PPE (after vox_ppe_assemble) to the training
+ * set; *index receives its number. */
+int vox_lpc_dump(vox_lpc* lpc, vox_ctx* ctx, int32_t* index);
+int vox_lpc_count(vox_lpc* lpc, int32_t* systems, int64_t* rows);
+/* Forward pass on stored system `index`: features feat[8*nu], scaling g[nu]
+ * and coupling c[nu] (any may be NULL). */
+int vox_lpc_eval(vox_lpc* lpc, int32_t index, float* feat, float* g, float* c);
+/* Loss and its gradient with respect to the parameters on stored system
+ * `index`, probes drawn from `seed`. */
+int vox_lpc_loss_grad(vox_lpc* lpc, int32_t index, uint64_t seed, double* loss, float* grad);
+/* Adam over every stored system, `epochs` passes, one step per system; mean
+ * loss of the first and of the last pass. */
+int vox_lpc_train(vox_lpc* lpc, int32_t epochs, double lr, uint64_t seed, double* loss_first,
+                  double* loss_last);
+/* Use `lpc` (NULL detaches) when the context's solver is VOX_SOLVER_LEARNED_FCG.
+ * The first learned solve runs Jacobi-CG to fix the baseline; a learned solve
+ * that needs more than twice the baseline's iterations, or breaks down, is
+ * redone by Jacobi-CG. */
+int vox_attach_lpc(vox_ctx* ctx, vox_lpc* lpc);
+
 /* ---- profiling ----------------------------------------------------------- */
 /* When enabled, HIP events bracket every launch of the named kernel classes on
  * the context's stream; vox_prof_get returns total ms and launch count. Class
  * names: cell, sort, list (list S search), list_l (list L search), pnd, sps,
  * vfrac, visc, pred, bcon, assemble, cg_init, spmv, cg_update, cg_p,
- * ir_residual, phat, pgrad, euler, collision, prep, misc. */
+ * ir_residual, phat, pgrad, euler, collision, prep, misc, lpc_net (features,
+ * network and coupling setup of a learned solve), lpc_apply (the
+ * preconditioner inside flexible CG). */
 int vox_prof_enable(vox_ctx* ctx, int on);
 int vox_prof_get(vox_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int vox_prof_reset(vox_ctx* ctx);

@@ -28,6 +28,8 @@ LIST_L = 1
 PREC_FP64 = 0
 PREC_MIXED = 1
 SOLVER_JACOBI_CG = 0
+SOLVER_LEARNED_FCG = 1
+LPC_NPARAM = 19330
 TEST_DAM, TEST_TURB, TEST_DROP = 0, 1, 2
 
 _i32 = C.c_int32
@@ -54,7 +56,7 @@ class VoxParams(C.Structure):
         ("cg_tol", _f64),
         ("cg_maxit", _i32), ("cg_check", _i32), ("cap_s", _i32), ("cap_l", _i32),
         ("ir_eta", _f64), ("ir_maxouter", _i32), ("warm_start", _i32),
-        ("list_mode", _i32), ("reserved1", _i32),
+        ("list_mode", _i32), ("lpc_cap_pct", _i32),
     ]
 
     def copy(self):
@@ -78,6 +80,7 @@ class VoxStatus(C.Structure):
         ("nan_count", _i32), ("out_of_box", _i32), ("max_count_s", _i32), ("max_count_l", _i32),
         ("collisions", _i32), ("collision_cand", _i32), ("steps", _i32), ("last_iters", _i32),
         ("last_relres", _f64), ("n0_s", _f64), ("n0_l", _f64), ("time", _f64),
+        ("lpc_fallbacks", _i32), ("lpc_baseline", _i32),
     ]
 
 
@@ -86,6 +89,7 @@ _ip = C.POINTER(C.c_int)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _pp = C.POINTER(VoxParams)
 
 # name -> (restype, argtypes)
@@ -142,6 +146,16 @@ PROTOTYPES = {
     "vox_grd_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 _i32p]),
     "vox_write_vtu": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "vox_lpc_create": (C.c_int, [C.POINTER(_P), C.c_uint64]),
+    "vox_lpc_destroy": (C.c_int, [_P]),
+    "vox_lpc_get_weights": (C.c_int, [_P, _fp]),
+    "vox_lpc_set_weights": (C.c_int, [_P, _fp]),
+    "vox_lpc_dump": (C.c_int, [_P, _P, _i32p]),
+    "vox_lpc_count": (C.c_int, [_P, _i32p, _i64p]),
+    "vox_lpc_eval": (C.c_int, [_P, C.c_int32, _fp, _fp, _fp]),
+    "vox_lpc_loss_grad": (C.c_int, [_P, C.c_int32, C.c_uint64, _dp, _fp]),
+    "vox_lpc_train": (C.c_int, [_P, C.c_int32, C.c_double, C.c_uint64, _dp, _dp]),
+    "vox_attach_lpc": (C.c_int, [_P, _P]),
 }
 
 

@@ -184,6 +184,26 @@ def box_of(case):
     return (case.x.min() - l0, case.x.max() + l0, case.y.min() - l0, case.y.max() + l0, 0.0, 0.0)
 
 
+def batch_cases(scen, name="batch"):
+    """Concatenate independent cases of one dimension, already placed apart
+    from each other, into one particle set ordered dummy -> wall -> fluid over
+    all of them, each keeping its own relative order. Returns (case, region,
+    boxes): region[i] is particle i's case, boxes the 6 box bounds of each case
+    run alone."""
+    groups = []
+    region = []
+    for sel in (lambda t: t < 0, lambda t: t == 0, lambda t: t > 0):
+        for k, c in enumerate(scen):
+            m = sel(c.ptype)
+            groups.append((c, m))
+            region.append(np.full(int(m.sum()), k, np.int32))
+    cat = lambda attr: np.concatenate([getattr(c, attr)[m] for c, m in groups])
+    case = Case(scen[0].dim, scen[0].l0, cat("x"), cat("y"), cat("z"), cat("u"), cat("v"),
+                cat("w"), cat("p"), cat("ptype").astype(np.int32), name=name)
+    boxes = np.array([box_of(c) for c in scen], np.float64)
+    return case, np.concatenate(region), boxes
+
+
 def ensemble_2d(widths, heights, l0=0.01, gap_cells=16):
     """Independent 2D two-phase dam breaks laid out along x in one particle set.
 
@@ -204,19 +224,23 @@ def ensemble_2d(widths, heights, l0=0.01, gap_cells=16):
         c = two_phase(dam_break(2, l0, (w, h), tank, 1, 2, name=f"col{w}x{h}", x_offset=off))
         scen.append(c)
         off += tank[0] + 2 + gap_cells
-    groups = []
-    region = []
-    for sel in (lambda t: t < 0, lambda t: t == 0, lambda t: t > 0):
-        for k, c in enumerate(scen):
-            m = sel(c.ptype)
-            groups.append((c, m))
-            region.append(np.full(int(m.sum()), k, np.int32))
-    cat = lambda attr: np.concatenate([getattr(c, attr)[m] for c, m in groups])
-    case = Case(2, l0, cat("x"), cat("y"), cat("z"), cat("u"), cat("v"), cat("w"), cat("p"),
-                cat("ptype").astype(np.int32), name=f"ensemble{len(scen)}",
-                meta={"widths": list(map(int, widths)), "heights": list(map(int, heights))})
-    boxes = np.array([box_of(c) for c in scen], np.float64)
-    return case, np.concatenate(region), boxes, scen
+    case, region, boxes = batch_cases(scen, name=f"ensemble{len(scen)}")
+    case.meta.update({"widths": list(map(int, widths)), "heights": list(map(int, heights))})
+    return case, region, boxes, scen
+
+
+def jittered_dam_breaks_2d(seeds, amplitude=0.1, gap_cells=16):
+    """The 2D dam break (l0 = 0.01, water 25 x 50, tank 160 x 60) once per
+    seed, each copy jittered by U[-amplitude*l0, amplitude*l0) from its seed
+    and placed `gap_cells` cells after the previous tank. Returns (case,
+    region, boxes, scenarios) as ensemble_2d does."""
+    scen = []
+    for k, seed in enumerate(seeds):
+        c = dam_break(2, 0.01, (25, 50), (160, 60), 1, 2, name=f"dam2d_seed{seed}",
+                      x_offset=k * (160 + 2 + gap_cells))
+        scen.append(jitter(c, amplitude * c.l0, seed=int(seed)))
+    case, region, boxes = batch_cases(scen, name=f"dam2d_x{len(scen)}")
+    return case, region, boxes, scen
 
 
 def ensemble_c4(count=64, seed=0, l0=0.01):

@@ -23,7 +23,7 @@ int fail(int code, const std::string& msg)
 static const char* kKNames[KC_COUNT] = {
     "cell", "sort", "list", "pnd", "sps", "vfrac", "visc", "pred", "bcon", "assemble",
     "cg_init", "spmv", "cg_update", "cg_p", "phat", "pgrad", "euler", "collision", "prep", "misc",
-    "ir_residual", "list_l"};
+    "ir_residual", "list_l", "lpc_net", "lpc_apply"};
 
 const char* kclass_name(int k) { return (k >= 0 && k < KC_COUNT) ? kKNames[k] : "?"; }
 int kclass_from_name(const char* name)
@@ -209,7 +209,7 @@ int vox_params_default(vox_params* p, int dim, double dl)
     p->ir_maxouter = 8;
     p->warm_start = 0;
     p->list_mode = 0;
-    p->reserved1 = 0;
+    p->lpc_cap_pct = 200;
     return VOX_OK;
 }
 
@@ -246,11 +246,13 @@ static int validate(const vox_params* p)
         return fail(VOX_ERR_ARG, "grid box is empty: call vox_params_box");
     if (p->precision != VOX_PREC_FP64 && p->precision != VOX_PREC_MIXED)
         return fail(VOX_ERR_NOLIB, "unknown precision mode");
-    if (p->solver != VOX_SOLVER_JACOBI_CG) return fail(VOX_ERR_NOLIB, "unknown PPE solver");
+    if (p->solver != VOX_SOLVER_JACOBI_CG && p->solver != VOX_SOLVER_LEARNED_FCG)
+        return fail(VOX_ERR_NOLIB, "unknown PPE solver");
     if (!(p->ir_eta >= 0.0 && p->ir_eta < 1.0) || p->ir_maxouter < 1 ||
         (p->warm_start != 0 && p->warm_start != 1))
         return fail(VOX_ERR_ARG, "ir_eta must be in [0, 1), ir_maxouter >= 1, warm_start 0 or 1");
     if (p->list_mode < 0 || p->list_mode > 2) return fail(VOX_ERR_ARG, "list_mode must be 0, 1 or 2");
+    if (p->lpc_cap_pct < 1) return fail(VOX_ERR_ARG, "lpc_cap_pct must be >= 1");
     if (p->ktype != 2 && p->ktype != 6 && p->ktype != 1 && p->ktype != 5)
         return fail(VOX_ERR_ARG, "unsupported kernel type");
     return VOX_OK;
@@ -309,7 +311,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_root, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
         (rc = dalloc(&c->coll_logn, 1)))
         return bail(rc);
-    if ((rc = dalloc(&c->partials, 8 * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
+    if ((rc = dalloc(&c->partials, kPartialSlots * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
         (rc = dalloc(&c->dflags, FLAG_COUNT)))
         return bail(rc);
     if (hipHostMalloc((void**)&c->hflags, sizeof(int32_t) * FLAG_COUNT, hipHostMallocDefault) != hipSuccess ||
@@ -342,6 +344,7 @@ int vox_destroy(vox_ctx* c)
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
     dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->region); dfree(c->rbox); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
     dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
+    lpc_free_ctx(c);
     if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto& r : c->prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }

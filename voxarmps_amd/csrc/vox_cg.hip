@@ -22,6 +22,17 @@
 // FP32 matrix values and FP32 vectors (FP64 dots) to a relative eta.
 //
 // The matrix is SELL-64 (mx_index); the product runs one thread per row.
+//
+// Flexible CG (solver VOX_SOLVER_LEARNED_FCG) takes z = M r from the learned
+// coupling operator (vox_lpc.hip) instead of r / d, and one more kernel per
+// iteration:
+//   K1 q = A p, partial p.q           (the product above)
+//   K2 x += alpha p; r -= alpha q     partial r.r
+//   K3 convergence test; z = M r      partials z.r and z.q
+//   K4 beta = -alpha z.q / rz (Polak-Ribiere form, since r_new - r_old =
+//      -alpha q); p = z + beta p
+// which stays a conjugate-direction method when M varies between solves or
+// is not exactly symmetric positive definite.
 #include <type_traits>
 
 #include "vox_internal.hpp"
@@ -31,11 +42,11 @@ namespace vox {
 constexpr int kCgBlocks = 2048;   // 8192 waves: 8 per SIMD on 256 CUs
 
 // partial slots (kMaxPartials doubles each)
-enum { P_BB = 0, P_RZ0, P_NNZ, P_PQ, P_RR, P_RZ, P_RES, P_RHS, P_COUNT };
+enum { P_BB = 0, P_RZ0, P_NNZ, P_PQ, P_RR, P_RZ, P_RES, P_RHS, P_ZQ, P_COUNT };
 // scalar slots
 enum { S_BNORM = 0, S_RZ0, S_RZ1, S_RELRES, S_NNZ, S_RESN, S_RHSN, S_COUNT_ };
 static_assert(S_COUNT_ <= SC_COUNT, "scalar slots");
-static_assert(P_COUNT <= 8, "partial slots");
+static_assert(P_COUNT <= kPartialSlots, "partial slots");
 
 template <class VT, class MT>
 struct CgArgs {
@@ -50,6 +61,12 @@ struct CgArgs {
     VT* r;
     VT* p;
     VT* q;
+    VT* z;                            // flexible CG: z = M r
+    const int32_t* __restrict__ slen; // coupling operator (SELL-64, cap scap)
+    const int32_t* __restrict__ scol;
+    const float* __restrict__ sval;
+    const float* __restrict__ gd;     // g_r / A_rr
+    int scap;
     double* part;
     double* scal;
     int32_t* flags;
@@ -220,6 +237,178 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<VT, MT> a)
         a.p[r] = a.r[r] / (VT)a.diag[r] + beta * a.p[r];
 }
 
+// ---- flexible CG ---------------------------------------------------------------------------
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_init(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    double bb = 0.0, nz = 0.0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
+        const double b = a.rhs[r];
+        a.x[r] = (VT)0;
+        a.r[r] = (VT)b;
+        bb += b * b;
+        nz += (double)(a.rowlen[r] + 1);
+    }
+    bb = block_sum(bb, lds);
+    nz = block_sum(nz, lds);
+    if (threadIdx.x == 0) {
+        PSLOT(a, P_BB)[blockIdx.x] = bb;
+        PSLOT(a, P_NNZ)[blockIdx.x] = nz;
+    }
+}
+
+// z = M r: g_r r_r / A_rr plus the coupling row; one thread per row.
+template <class VT, class MT>
+__device__ __forceinline__ VT apply_m(const CgArgs<VT, MT>& a, int r)
+{
+    VT acc = (VT)a.gd[r] * a.r[r];
+    const long b = mx_index(r, 0, a.scap);
+    const int len = a.slen[r];
+    constexpr int U = 8;
+    int k = 0;
+    for (; k + U <= len; k += U) {
+        int32_t cc[U];
+        float vv[U];
+        VT rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cc[u] = __builtin_nontemporal_load(&a.scol[b + 64L * (k + u)]);
+            vv[u] = __builtin_nontemporal_load(&a.sval[b + 64L * (k + u)]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) rr[u] = a.r[cc[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc = acc + (VT)vv[u] * rr[u];
+    }
+    for (; k < len; ++k) acc = acc + (VT)a.sval[b + 64L * k] * a.r[a.scol[b + 64L * k]];
+    return acc;
+}
+
+// first direction: z = M b, p = z, partial z.b
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_prec0(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    double zr = 0.0;
+    const int r = xcd_tid();
+    if (r < a.nu) {
+        const VT z = apply_m(a, r);
+        a.z[r] = z;
+        a.p[r] = z;
+        zr = (double)z * (double)a.r[r];
+    }
+    zr = block_sum(zr, lds);
+    if (threadIdx.x == 0) PSLOT(a, P_RZ0)[blockIdx.x] = zr;
+}
+
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_init_fin(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    const double bb = sum_partials(PSLOT(a, P_BB), kCgBlocks, lds);
+    const double nz = sum_partials(PSLOT(a, P_NNZ), kCgBlocks, lds);
+    const double rz = sum_partials(PSLOT(a, P_RZ0), a.npq, lds);
+    if (threadIdx.x == 0) {
+        const double bn = sqrt(bb);
+        a.scal[S_BNORM] = bn;
+        a.scal[S_RZ0] = rz;
+        a.scal[S_NNZ] = nz;
+        a.scal[S_RELRES] = 0.0;
+        a.flags[FLAG_CG_ITERS] = 0;
+        const bool zero = (bn == 0.0);
+        const bool brk = !zero && !(rz > 0.0 && isfinite(rz));
+        a.flags[FLAG_CG_BREAK] = brk ? 1 : 0;
+        a.flags[FLAG_CG_CONV] = zero ? 1 : 0;
+        a.flags[FLAG_CG_DONE] = (zero || brk) ? 1 : 0;
+    }
+}
+
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_update(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    if (a.flags[FLAG_CG_DONE]) return;
+    const double pq = sum_partials(PSLOT(a, P_PQ), a.npq, lds);
+    const double rz_old = a.scal[S_RZ0 + ((a.it - 1) & 1)];
+    if (!(pq > 0.0) || !isfinite(pq)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.flags[FLAG_CG_BREAK] = 1;
+            a.flags[FLAG_CG_DONE] = 1;
+            a.flags[FLAG_CG_ITERS] = a.it - 1;
+        }
+        return;
+    }
+    const VT alpha = (VT)(rz_old / pq);
+    double rr = 0.0;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
+        a.x[r] = a.x[r] + alpha * a.p[r];
+        const VT res = a.r[r] - alpha * a.q[r];
+        a.r[r] = res;
+        rr += (double)res * (double)res;
+    }
+    rr = block_sum(rr, lds);
+    if (threadIdx.x == 0) PSLOT(a, P_RR)[blockIdx.x] = rr;
+}
+
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_prec(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    if (a.flags[FLAG_CG_DONE]) return;
+    const double rr = sum_partials(PSLOT(a, P_RR), kCgBlocks, lds);
+    const double rel = sqrt(rr) / a.scal[S_BNORM];
+    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    if (lead) {
+        a.scal[S_RELRES] = rel;
+        a.flags[FLAG_CG_ITERS] = a.it;
+    }
+    if (rel <= a.tol) {
+        if (lead) { a.flags[FLAG_CG_CONV] = 1; a.flags[FLAG_CG_DONE] = 1; }
+        return;
+    }
+    if (a.it >= a.maxit || !isfinite(rel)) {
+        if (lead) a.flags[FLAG_CG_DONE] = 1;
+        return;
+    }
+    double zr = 0.0, zq = 0.0;
+    const int r = xcd_tid();
+    if (r < a.nu) {
+        const VT z = apply_m(a, r);
+        a.z[r] = z;
+        zr = (double)z * (double)a.r[r];
+        zq = (double)z * (double)a.q[r];
+    }
+    zr = block_sum(zr, lds);
+    zq = block_sum(zq, lds);
+    if (threadIdx.x == 0) {
+        PSLOT(a, P_RZ)[blockIdx.x] = zr;
+        PSLOT(a, P_ZQ)[blockIdx.x] = zq;
+    }
+}
+
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_fcg_p(CgArgs<VT, MT> a)
+{
+    __shared__ double lds[kBlock];
+    if (a.flags[FLAG_CG_DONE]) return;
+    const double pq = sum_partials(PSLOT(a, P_PQ), a.npq, lds);
+    const double zr = sum_partials(PSLOT(a, P_RZ), a.npq, lds);
+    const double zq = sum_partials(PSLOT(a, P_ZQ), a.npq, lds);
+    const double rz_old = a.scal[S_RZ0 + ((a.it - 1) & 1)];
+    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    if (!(zr > 0.0) || !isfinite(zr)) {
+        // M r is not a descent direction: the preconditioner is not positive here
+        if (lead) { a.flags[FLAG_CG_BREAK] = 1; a.flags[FLAG_CG_DONE] = 1; }
+        return;
+    }
+    const double alpha = rz_old / pq;
+    const VT beta = (VT)(-alpha * zq / rz_old);
+    if (lead) a.scal[S_RZ0 + (a.it & 1)] = zr;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x)
+        a.p[r] = a.z[r] + beta * a.p[r];
+}
+
 // ---- outer FP64 refinement kernels ---------------------------------------------------------
 struct IrArgs {
     int nu, cap, npq;
@@ -356,6 +545,58 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
     return VOX_OK;
 }
 
+// Flexible CG with the learned coupling operator; same check schedule as run_cg.
+template <class VT, class MT>
+static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
+{
+    {
+        ProfScope ps(c, KC_CG_INIT);
+        k_fcg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+    }
+    {
+        ProfScope ps(c, KC_LPC_APPLY);
+        k_fcg_prec0<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
+    }
+    {
+        ProfScope ps(c, KC_CG_INIT);
+        k_fcg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
+        VOX_HIP(hipGetLastError());
+    }
+    const int check = c->p.cg_check > 0 ? c->p.cg_check : 1;
+    int& expect = std::is_same<VT, double>::value ? c->fcg_expect : c->fcg_expect_inner;
+    const int first = expect - 2;
+    int it = 0;
+    VOX_TRY(read_flags(c));
+    c->nnz_total = (long)c->hscal[S_NNZ];
+    while (!c->hflags[FLAG_CG_DONE] && it < a.maxit) {
+        const int batch = (it == 0 && first > check) ? first : check;
+        for (int k = 0; k < batch && it < a.maxit; ++k) {
+            a.it = ++it;
+            {
+                ProfScope ps(c, KC_SPMV);
+                k_cg_spmv<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
+            }
+            {
+                ProfScope ps(c, KC_CG_UPDATE);
+                k_fcg_update<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+            }
+            {
+                ProfScope ps(c, KC_LPC_APPLY);
+                k_fcg_prec<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
+            }
+            {
+                ProfScope ps(c, KC_CG_P);
+                k_fcg_p<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+            }
+        }
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(read_flags(c));
+    }
+    expect = c->hflags[FLAG_CG_ITERS];
+    *iters_out = c->hflags[FLAG_CG_ITERS];
+    return VOX_OK;
+}
+
 template <class VT, class MT>
 static void fill_args(vox_ctx* c, CgArgs<VT, MT>& a, const MT* aval, VT* x, VT* r, VT* p, VT* q,
                       const double* rhs, double tol, int maxit)
@@ -365,6 +606,8 @@ static void fill_args(vox_ctx* c, CgArgs<VT, MT>& a, const MT* aval, VT* x, VT* 
     a.rowlen = c->rowlen; a.acol = c->acol; a.aval = aval; a.diag = c->adiag; a.rhs = rhs;
     a.x = x; a.r = r; a.p = p; a.q = q;
     a.part = c->partials; a.scal = c->dscal; a.flags = c->dflags;
+    a.z = nullptr; a.slen = c->lpc_slen; a.scol = c->lpc_scol; a.sval = c->lpc_sval; a.gd = c->lpc_gd;
+    a.scap = c->lpc_scap;
 }
 
 static int finish(vox_ctx* c, vox_solve_info* info, int iters, int conv, double relres,
@@ -393,14 +636,16 @@ static int finish(vox_ctx* c, vox_solve_info* info, int iters, int conv, double 
     return VOX_OK;
 }
 
-static int solve_fp64(vox_ctx* c, vox_solve_info* info)
+static int solve_fp64(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 {
     CgArgs<double, double> a;
     fill_args(c, a, (const double*)c->aval, c->cg_x, c->cg_r, c->cg_p, c->cg_q, c->rhs,
-              c->p.cg_tol, c->p.cg_maxit);
+              c->p.cg_tol, maxit);
+    a.z = c->cg_z;
     if (a.npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
     int iters = 0;
-    VOX_TRY(run_cg(c, a, &iters));
+    if (fcg) VOX_TRY(run_fcg(c, a, &iters));
+    else VOX_TRY(run_cg(c, a, &iters));
     return finish(c, info, iters, c->hflags[FLAG_CG_CONV], c->hscal[S_RELRES], c->hscal[S_BNORM],
                   c->hflags[FLAG_CG_BREAK]);
 }
@@ -413,7 +658,7 @@ static int solve_fp64(vox_ctx* c, vox_solve_info* info)
 // the target.
 template <class VT, class MT>
 static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x, VT* r, VT* p,
-                         VT* q, double eta, bool warm)
+                         VT* q, VT* z, double eta, bool warm, bool fcg, int maxit)
 {
     const int nu = c->nu;
     IrArgs ir;
@@ -446,16 +691,18 @@ static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x
             bnorm = c->hscal[S_RHSN];
             rel = (bnorm > 0) ? rn / bnorm : 0.0;
             if (rel <= tol) { conv = 1; break; }
-            if (outer >= c->p.ir_maxouter || total >= c->p.cg_maxit || !std::isfinite(rel)) break;
+            if (outer >= c->p.ir_maxouter || total >= maxit || !std::isfinite(rel)) break;
         }
         // relative target of this correction: no tighter than the remaining
         // factor to the goal (half of it, for the rounding of the update)
         const double need = 0.5 * tol / rel;
         CgArgs<VT, MT> a;
         fill_args(c, a, aval, x, r, p, q, from_b ? c->rhs : c->cg_r, eta > need ? eta : need,
-                  c->p.cg_maxit - total);
+                  maxit - total);
+        a.z = z;
         int it = 0;
-        VOX_TRY(run_cg(c, a, &it));
+        if (fcg) VOX_TRY(run_fcg(c, a, &it));
+        else VOX_TRY(run_cg(c, a, &it));
         total += it;
         if (c->hflags[FLAG_CG_BREAK] && it == 0) { brk = 1; break; }
         k_ir_correct<VT><<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->cg_x, x);
@@ -465,21 +712,53 @@ static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x
     return finish(c, info, total, conv, rel, bnorm, brk);
 }
 
-static int solve_mixed(vox_ctx* c, vox_solve_info* info)
+static int solve_mixed(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 {
     const double eta = c->p.ir_eta > 0 ? c->p.ir_eta : 1e-4;
+    if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
     return solve_refined<float, float>(c, info, (const float*)c->aval32, c->cg_x32, c->cg_r32,
-                                       c->cg_p32, c->cg_q32, eta, c->p.warm_start != 0);
+                                       c->cg_p32, c->cg_q32, c->cg_z32, eta, c->p.warm_start != 0,
+                                       fcg, maxit);
+}
+
+static int solve_plain(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
+{
+    VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));
+    if (c->p.precision == VOX_PREC_MIXED) return solve_mixed(c, info, fcg, maxit);
+    if (c->p.warm_start)
+        return solve_refined<double, double>(c, info, (const double*)c->aval, c->cg_d, c->cg_e,
+                                             c->cg_p, c->cg_q, c->cg_z, 0.0, true, fcg, maxit);
+    return solve_fp64(c, info, fcg, maxit);
+}
+
+// Learned solve: flexible CG with the network's preconditioner, held to twice
+// the iterations of the last Jacobi-CG solve (the baseline); a first solve
+// without a baseline, a solve over the cap and a breakdown run Jacobi-CG,
+// which is also the judge of every result (same tolerance on the same system).
+static int solve_learned(vox_ctx* c, vox_solve_info* info)
+{
+    if (c->lpc_baseline <= 0) {
+        const int rc = solve_plain(c, info, false, c->p.cg_maxit);
+        if (rc == VOX_OK) c->lpc_baseline = c->st.last_iters > 0 ? c->st.last_iters : 1;
+        c->st.lpc_baseline = c->lpc_baseline;
+        return rc;
+    }
+    VOX_TRY(lpc_prepare(c));
+    const long want = ((long)c->lpc_baseline * c->p.lpc_cap_pct + 99) / 100;
+    const int cap = (int)std::max(1L, std::min((long)c->p.cg_maxit, want));
+    int rc = solve_plain(c, info, true, cap);
+    if (rc != VOX_ERR_SOLVER) return rc;
+    c->st.lpc_fallbacks += 1;
+    rc = solve_plain(c, info, false, c->p.cg_maxit);
+    if (rc == VOX_OK) c->lpc_baseline = c->st.last_iters > 0 ? c->st.last_iters : 1;
+    c->st.lpc_baseline = c->lpc_baseline;
+    return rc;
 }
 
 int ppe_solve(vox_ctx* c, vox_solve_info* info)
 {
-    VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));
-    if (c->p.precision == VOX_PREC_MIXED) return solve_mixed(c, info);
-    if (c->p.warm_start)
-        return solve_refined<double, double>(c, info, (const double*)c->aval, c->cg_d, c->cg_e,
-                                             c->cg_p, c->cg_q, 0.0, true);
-    return solve_fp64(c, info);
+    if (c->p.solver == VOX_SOLVER_LEARNED_FCG) return solve_learned(c, info);
+    return solve_plain(c, info, false, c->p.cg_maxit);
 }
 
 }  // namespace vox

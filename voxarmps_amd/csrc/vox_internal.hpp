@@ -17,6 +17,7 @@ namespace vox {
 constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
+constexpr int kPartialSlots = 10;    // scalars reduced through ctx->partials
 
 
 // Gather records: everything a neighbour loop reads about particle j in one
@@ -214,6 +215,8 @@ struct vox_ctx {
     long nnz_total = 0;            // stored entries incl. diagonal (last solve)
     int cg_expect = 0;             // iterations of the previous FP64 solve
     int cg_expect_inner = 0;       // iterations of the previous FP32 inner solve
+    int fcg_expect = 0;            // the same for flexible CG
+    int fcg_expect_inner = 0;
     int ir_outer_last = 0;
     // CG vectors over unknowns
     double* cg_x = nullptr;
@@ -224,10 +227,33 @@ struct vox_ctx {
     double* cg_e = nullptr;        //   and residual
     // mixed precision: FP32 matrix values and inner-solve vectors
     float* aval32 = nullptr;       // same sliced layout as aval
+    bool aval32_ok = false;        // aval32 holds the current assembly's values
     float* cg_x32 = nullptr;
     float* cg_r32 = nullptr;
     float* cg_p32 = nullptr;
     float* cg_q32 = nullptr;
+
+    // learned preconditioner: attached network, per-unknown buffers of its
+    // forward pass and the coupling operator of flexible CG
+    vox_lpc* lpc = nullptr;
+    int lpc_baseline = 0;          // Jacobi-CG iterations of the last plain solve
+    int lpc_rows = 0;              // rows the buffers below hold (multiple of 64)
+    float* lpc_feat = nullptr;     // 8 per row
+    float* lpc_inv_osum = nullptr;
+    float* lpc_invd = nullptr;     // 1 / A_rr
+    float* lpc_t = nullptr;        // sqrt(c_r) / A_rr
+    vox::v4f* lpc_upos = nullptr;  // FP32 position of each unknown
+    uint16_t* lpc_h[2] = {};       // bf16 hidden states, 64 per row (ping-pong)
+    uint16_t* lpc_s = nullptr;     // bf16 messages, 16 per row
+    uint16_t* lpc_m = nullptr;     // bf16 aggregated messages, 16 per row
+    float* lpc_o = nullptr;        // network output, 2 per row
+    float* lpc_gd = nullptr;       // g_r / A_rr
+    int32_t* lpc_slen = nullptr;   // coupling operator, SELL-64 with cap a_cap
+    int32_t* lpc_scol = nullptr;
+    float* lpc_sval = nullptr;
+    int lpc_scap = 0;
+    double* cg_z = nullptr;        // flexible CG: preconditioned residual
+    float* cg_z32 = nullptr;
 
     // collision exact-emulation buffers
     int32_t* coll_cnt = nullptr;   // tp
@@ -276,7 +302,8 @@ namespace vox {
 enum KClass {
     KC_CELL = 0, KC_SORT, KC_LIST, KC_PND, KC_SPS, KC_VFRAC, KC_VISC, KC_PRED,
     KC_BCON, KC_ASSEMBLE, KC_CG_INIT, KC_SPMV, KC_CG_UPDATE, KC_CG_P, KC_PHAT,
-    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_IR, KC_LIST_L, KC_COUNT
+    KC_PGRAD, KC_EULER, KC_COLL, KC_PREP, KC_MISC, KC_IR, KC_LIST_L, KC_LPC_NET, KC_LPC_APPLY,
+    KC_COUNT
 };
 const char* kclass_name(int k);
 int kclass_from_name(const char* name);
@@ -334,6 +361,12 @@ int ppe_assemble(vox_ctx* c);
 int ppe_solve(vox_ctx* c, vox_solve_info* info);
 int ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32_t* cols,
                double* vals, double* rhs, int32_t* ids);
+
+// learned preconditioner (vox_lpc.hip): features, network and coupling
+// operator for the context's assembled PPE
+int lpc_prepare(vox_ctx* c);
+int ppe_make_aval32(vox_ctx* c);   // FP32 copy of the assembled values (vox_ppe.hip)
+void lpc_free_ctx(vox_ctx* c);
 
 }  // namespace vox
 

@@ -216,11 +216,13 @@ int ppe_assemble(vox_ctx* c)
         VOX_HIP(hipMalloc(&c->tval, sizeof(double) * (size_t)sl_size(n, L.cap)));
         if (c->aval32) (void)hipFree(c->aval32);
         c->aval32 = nullptr;
+        c->aval32_ok = false;
         c->a_cap = L.cap;
     }
-    const bool mixed = (c->p.precision == VOX_PREC_MIXED);
-    if (mixed && !c->aval32)
-        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(n, c->a_cap)));
+    // FP32 values: the mixed-precision inner solves and the learned
+    // preconditioner's network read them
+    const bool mixed = (c->p.precision == VOX_PREC_MIXED) ||
+                       (c->p.solver == VOX_SOLVER_LEARNED_FCG && c->lpc);
     k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
     VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
     k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
@@ -253,11 +255,21 @@ int ppe_assemble(vox_ctx* c)
                                                                    c->aval);
         VOX_HIP(hipGetLastError());
     }
-    if (mixed && c->nu > 0) {
+    c->aval32_ok = false;
+    if (mixed) VOX_TRY(ppe_make_aval32(c));
+    return VOX_OK;
+}
+
+int ppe_make_aval32(vox_ctx* c)
+{
+    if (!c->aval32)
+        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
+    if (c->nu > 0) {
         k_aval32<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen, c->aval,
                                                            c->aval32);
         VOX_HIP(hipGetLastError());
     }
+    c->aval32_ok = true;
     return VOX_OK;
 }
 

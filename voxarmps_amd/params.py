@@ -51,6 +51,7 @@ def default_params(dim, dl):
     p.ir_maxouter = 8
     p.warm_start = 0
     p.list_mode = 0
+    p.lpc_cap_pct = 200
     return p
 
 

@@ -87,6 +87,7 @@ class Simulation:
     # ---- lifetime -----------------------------------------------------------
     def close(self):
         if self.ctx:
+            self._lpc = None
             self.be.fn("destroy")(self.ctx)
             self.ctx = C.c_void_p()
 
@@ -236,6 +237,13 @@ class Simulation:
         m = np.zeros(2, np.int32)
         self.be.check(self.be.fn("autotune")(self.ctx, int(reps), _ptr(m, C.c_int32)), "autotune")
         return int(m[0]), int(m[1])
+
+    def attach_lpc(self, lpc):
+        """Use a LearnedPreconditioner when the solver is the learned FCG
+        (None detaches). The simulation keeps a reference to it."""
+        self.be.check(self.be.fn("attach_lpc")(self.ctx, lpc.h if lpc is not None else None),
+                      "attach_lpc")
+        self._lpc = lpc
 
     def ppe(self):
         """Assembled PPE as CSR over the bcon == 0 rows: dict with rowptr, cols,
