@@ -123,6 +123,14 @@ typedef struct vox_params {
                              2 one wave per grid cell (same lists either way)    */
     int32_t lpc_cap_pct;  /* learned FCG: iteration cap in percent of the Jacobi-CG
                              baseline before the fallback (200: twice)         */
+    int32_t method;       /* PRESSURECALC: 1 PPE (incompressible), 3 weakly
+                             compressible equation of state (main.cu:335-339)  */
+    int32_t matopt;       /* 1: higher-order Laplacian in MATRIX (functions.cu:901-914) */
+    int32_t srcopt;       /* 1: higher-order source with error compensation
+                             (functions.cu:928-988)                              */
+    int32_t reserved2;
+    double  wc_beta;      /* method 3: free-surface PND ratio of the equation of
+                             state (BETA, main.cu:262)                           */
 } vox_params;
 
 /* PPE solve report */
@@ -202,14 +210,16 @@ int vox_viscosity(vox_ctx* ctx);
 int vox_prediction(vox_ctx* ctx);
 /* BCON (functions.cuh:97, functions.cu:433-445). */
 int vox_bcon(vox_ctx* ctx);
-/* MATRIX, Method 1 without matopt/srcopt (functions.cuh:62, functions.cu:869-1005):
- * device CSR over the bcon == 0 rows plus right-hand side. */
+/* MATRIX, Method 1 (functions.cuh:62, functions.cu:869-1005), with the
+ * higher-order Laplacian (matopt) and source (srcopt) when set: device matrix
+ * over the bcon == 0 rows plus right-hand side. */
 int vox_ppe_assemble(vox_ctx* ctx);
 /* Solve the assembled PPE into pnew from a zero start (replaces INCDECOM + CGM,
  * functions.cu:1139-1200, 506-716). */
 int vox_ppe_solve(vox_ctx* ctx, vox_solve_info* info);
-/* PRESSURECALC, Method 1 (functions.cuh:101, functions.cu:2465-2549):
- * pnew = 0, assemble, solve, clamp to [pmin, pmax] with NaN -> pmax. */
+/* PRESSURECALC (functions.cuh:101, functions.cu:2465-2549). Method 1: pnew =
+ * 0, assemble, solve, clamp to [pmin, pmax] with NaN -> pmax. Method 3: the
+ * equation of state on n* for non-dummy particles (2471-2500). */
 int vox_pressurecalc(vox_ctx* ctx, vox_solve_info* info);
 /* PHATCALC (functions.cuh:69, functions.cu:207-224). */
 int vox_phat(vox_ctx* ctx);

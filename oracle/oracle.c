@@ -636,7 +636,15 @@ int orc_ppe_assemble(orc_ctx* c)
             if (c->bcon[j] == -1) { a[l] = 0.0; continue; }
             double d = dist(c, x, y, z, i, j);
             if (fabs(d) <= 1.0e-15) continue;
-            double val = 2.0 * p->dim / p->lambda * wfun(d, p->ktype, p->re_l) / c->n0l / p->rho1;
+            double val;
+            if (p->matopt) {                                           /* 901-914 */
+                val = (p->dim == 2) ? 3.0 * p->re_l : 2.0 * p->re_l;
+                val = val / (d * d * d);
+                val = val / c->n0l;
+                val = val / p->rho1;
+            } else {
+                val = 2.0 * p->dim / p->lambda * wfun(d, p->ktype, p->re_l) / c->n0l / p->rho1;
+            }
             a[l] = -val;
             diag += val;
         }
@@ -644,6 +652,45 @@ int orc_ppe_assemble(orc_ctx* c)
         c->adiag[i] = diag;
     }
     memset(src, 0, sizeof(double) * tp);                                /* 2512 */
+    if (p->srcopt) {
+        /* 928-988, sequentially: source[i-1] is the value this loop has just
+         * written (0 for particles it skips or sets to 0). The velocity
+         * divergence uses the Laplacian list and n0, the PND terms n0 of n*. */
+        const double* ns = c->f[VOX_F_NSTAR];
+        const double *us = c->f[VOX_F_USTAR], *vs = c->f[VOX_F_VSTAR], *ws = c->f[VOX_F_WSTAR];
+        const double re = p->re_l, n0l = c->n0l, n0s = c->n0s;
+        for (int i = 0; i < tp; ++i) {
+            if (c->ptype[i] == -1) continue;
+            if (c->bcon[i] == 0) {
+                double alpha = 0.0, beta = 0.0, ecs = 0.0;
+                if (i > 0) {
+                    alpha = (ns[i] - n0s) / n0s;
+                    beta = (dt / n0s) * (src[i - 1] / (-1.0 / (n0s * dt)));
+                    ecs = fabs(alpha) * (beta / dt) + fabs(beta) * (alpha / dt);
+                }
+                double sum = 0.0;
+                const int32_t* r = ROW(L, i);
+                for (int l = 0; l < L->cnt[i]; ++l) {
+                    int j = r[l];
+                    if (c->ptype[j] <= -1 || i == j) continue;
+                    double d = dist(c, x, y, z, i, j);
+                    if (fabs(d) <= 1.0e-15) continue;
+                    double du = us[i] - us[j], dv = vs[i] - vs[j];
+                    if (p->dim == 3) {
+                        double dw = ws[i] - ws[j];
+                        sum += (re / (d * d * d)) * ((x[i] - x[j]) * du + (y[i] - y[j]) * dv + (z[i] - z[j]) * dw);
+                    } else {
+                        sum += (re / (d * d * d)) * ((x[i] - x[j]) * du + (y[i] - y[j]) * dv);
+                    }
+                }
+                src[i] = (-1.25 / (n0l * dt)) * sum + ecs;
+                if (fabs(src[i]) < 0.99) src[i] = 0;
+            } else if (c->bcon[i] == 1) {
+                src[i] = 0.0;
+            }
+        }
+        return VOX_OK;
+    }
     for (int i = 0; i < tp; ++i) {                                      /* 990-1001 */
         if (c->ptype[i] == -1) continue;
         if (c->bcon[i] == 0) src[i] = 1.0 / dt / dt * (n[i] - c->n0s) / c->n0s;
@@ -744,6 +791,21 @@ int orc_ppe_solve(orc_ctx* c, vox_solve_info* info)
 int orc_pressurecalc(orc_ctx* c, vox_solve_info* info)
 {
     double* pn = c->f[VOX_F_PNEW];
+    if (c->p.method == 3) {                                   /* 2471-2500 */
+        const double* ns = c->f[VOX_F_NSTAR];
+        const double* C = c->f[VOX_F_C];
+        const vox_params* p = &c->p;
+        for (int i = c->gp; i < c->tp; ++i) {
+            const double c0 = (c->ptype[i] == 2) ? p->c02 : p->c01;
+            const double rho = p->rho2 * C[i] + p->rho1 * (1 - C[i]);
+            if (ns[i] < p->wc_beta * c->n0s) pn[i] = 0.0;
+            else pn[i] = (c0 * c0 * rho / 7.0) * (pow(ns[i] / c->n0s, 7.0) - 1);
+            if (pn[i] < p->pmin) pn[i] = p->pmin;
+            if (pn[i] > p->pmax) pn[i] = p->pmax;
+        }
+        if (info) memset(info, 0, sizeof(*info));
+        return VOX_OK;
+    }
     memset(pn, 0, sizeof(double) * c->tp);                  /* 2503 */
     int rc = orc_ppe_assemble(c);
     if (rc) return rc;
@@ -954,14 +1016,14 @@ int orc_step(orc_ctx* c, int nsteps)
     for (int s = 0; s < nsteps; ++s) {
         int rc = VOX_OK;
         if (c->p.dt_mode == 1) orc_dtcalc(c);
-        if (!rc) rc = orc_pnum(c, 1);
+        if (!rc && c->p.method != 3) rc = orc_pnum(c, 1);     /* 1036: not in method 3 */
         if (!rc) rc = orc_neighbor(c);
         if (!rc && c->p.turb) rc = orc_sps(c);
         if (!rc) rc = orc_vfraction(c);
         if (!rc) rc = orc_viscosity(c);
         if (!rc) rc = orc_prediction(c);
         if (!rc) rc = orc_pnumstar(c);
-        if (!rc) rc = orc_bcon(c);
+        if (!rc && c->p.method != 3) rc = orc_bcon(c);        /* 1133 */
         if (!rc) rc = orc_pressurecalc(c, NULL);
         if (!rc) rc = orc_phat(c);
         if (!rc) rc = orc_pressgrad(c);

@@ -163,6 +163,23 @@ def test_stages_bit_exact(hip, oracle, name, case, p):
     Ag, Ac = g.ppe(), c.ppe()
     for k in ("rowptr", "cols", "vals", "rhs", "ids"):
         assert np.array_equal(Ag[k], Ac[k]), f"{name} MATRIX {k}"
+    # MATRIX options (functions.cu:901-914, 928-988): higher-order Laplacian
+    # and the higher-order, error-compensated source, bit-exact as well
+    for opt in ("matopt", "srcopt"):
+        q = p.copy()
+        setattr(q, opt, 1)
+        for sim in (g, c):
+            sim.set_params(q)
+            sim.MATRIX()
+        _assert_same(g, c, ["SOURCE"], f"{name} MATRIX {opt} source")
+        Ag, Ac = g.ppe(), c.ppe()
+        for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+            assert np.array_equal(Ag[k], Ac[k]), f"{name} MATRIX {opt} {k}"
+    src = g.get("SOURCE")
+    assert np.count_nonzero(src) > 0
+    for sim in (g, c):
+        sim.set_params(p)
+        sim.MATRIX()
     ig = g.CGM()
     ic = c.CGM()
     assert ig.converged and ic.converged
@@ -170,6 +187,25 @@ def test_stages_bit_exact(hip, oracle, name, case, p):
     assert abs(ig.iterations - ic.iterations) <= 1
     pg, pc = g.get("PNEW"), c.get("PNEW")
     assert np.max(np.abs(pg - pc)) <= 1e-6 * max(np.max(np.abs(pc)), 1.0)
+    # PRESSURECALC method 3, the equation of state on n* (functions.cu:2471-2500):
+    # pow(n*/n0, 7) may differ in its last bits between the device and host
+    # libraries; (n*/n0)^7 - 1 cancels near n* = n0, so the bound is absolute,
+    # 16 ulps of c0^2 rho / 7
+    q = p.copy()
+    q.method = 3
+    nstar = c.get("NSTAR")
+    ratio = np.random.default_rng(11).uniform(0.85, 1.05, case.tp)   # both sides of 0.92
+    for sim in (g, c):
+        sim.set("NSTAR", ratio * c.n0()[0])
+        sim.set_params(q)
+        sim.PRESSURECALC()
+    pg, pc = g.get("PNEW"), c.get("PNEW")
+    coef = max(q.c01, q.c02) ** 2 * max(q.rho1, q.rho2) / 7.0
+    assert np.max(np.abs(pg - pc)) <= 16 * np.finfo(float).eps * coef
+    assert np.count_nonzero(pc[case.ptype >= 0]) > 0
+    for sim in (g, c):
+        sim.set("NSTAR", nstar)
+        sim.set_params(p)
     # downstream stages from identical pressure
     _sync_fields(c, g, ["PNEW"])
     for st in ("PHATCALC", "PRESSGRAD", "EULERINTEGRATION"):
@@ -280,6 +316,28 @@ def test_trajectory_2d(hip, oracle, nsteps):
         a, b = g.get(n), c.get(n)
         scale = max(np.max(np.abs(b)), 1.0)
         assert np.max(np.abs(a - b)) <= 1e-6 * scale, n
+    _assert_lists(g, c, "trajectory end")
+    g.close()
+    c.close()
+
+
+def test_trajectory_weakly_compressible(hip, oracle):
+    """PRESSURECALC method 3 (equation of state, no PPE) with its parameter set
+    (cubic kernel, relaxp 0.5, collision coefficient 0.5): 20 steps of the 2D
+    dam break. Only pow() can differ (last bits), so the trajectories agree to
+    1e-9 of each field's scale."""
+    case = vx.dam_break_2d_c0()
+    p = vx.wc_params(case)
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    g.step(20)
+    c.step(20)
+    for n in ("X", "Y", "U", "V", "PNEW"):
+        a, b = g.get(n), c.get(n)
+        scale = max(np.max(np.abs(b)), 1.0)
+        assert np.max(np.abs(a - b)) <= 1e-9 * scale, n
+    assert np.max(c.get("PNEW")) > 0
     _assert_lists(g, c, "trajectory end")
     g.close()
     c.close()

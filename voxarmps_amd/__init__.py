@@ -7,7 +7,7 @@ from ._abi import F, FIELDS, LIST_L, LIST_S, VoxParams, VoxSolveInfo, VoxStatus 
 from .cases import (Case, batch_cases, box_of, dam_break, dam_break_2d_c0, dam_break_3d_c1,  # noqa: F401
                     ensemble_2d, ensemble_c4, jitter, jittered_dam_breaks_2d, two_phase)
 from .params import (dam2d_params, dam3d_params, default_params, ensemble_params,  # noqa: F401
-                     set_box, two_radius)
+                     set_box, two_radius, wc_params)
 from .sim import Backend, Simulation, VoxError, hip_backend  # noqa: F401
 from .lpc import LearnedPreconditioner  # noqa: F401
 

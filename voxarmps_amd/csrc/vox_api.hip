@@ -210,6 +210,11 @@ int vox_params_default(vox_params* p, int dim, double dl)
     p->warm_start = 0;
     p->list_mode = 0;
     p->lpc_cap_pct = 200;
+    p->method = 1;
+    p->matopt = 0;
+    p->srcopt = 0;
+    p->reserved2 = 0;
+    p->wc_beta = 0.92;                              // main.cu:262
     return VOX_OK;
 }
 
@@ -253,6 +258,10 @@ static int validate(const vox_params* p)
         return fail(VOX_ERR_ARG, "ir_eta must be in [0, 1), ir_maxouter >= 1, warm_start 0 or 1");
     if (p->list_mode < 0 || p->list_mode > 2) return fail(VOX_ERR_ARG, "list_mode must be 0, 1 or 2");
     if (p->lpc_cap_pct < 1) return fail(VOX_ERR_ARG, "lpc_cap_pct must be >= 1");
+    if (p->method != 1 && p->method != 3)
+        return fail(VOX_ERR_NOLIB, "PRESSURECALC method must be 1 (PPE) or 3 (equation of state)");
+    if ((p->matopt != 0 && p->matopt != 1) || (p->srcopt != 0 && p->srcopt != 1))
+        return fail(VOX_ERR_ARG, "matopt and srcopt must be 0 or 1");
     if (p->ktype != 2 && p->ktype != 6 && p->ktype != 1 && p->ktype != 5)
         return fail(VOX_ERR_ARG, "unsupported kernel type");
     return VOX_OK;
@@ -457,6 +466,10 @@ int vox_ppe_solve(vox_ctx* c, vox_solve_info* info)
 int vox_pressurecalc(vox_ctx* c, vox_solve_info* info)
 {
     VOX_TRY(need_init(c));
+    if (c->p.method == 3) {                                 // functions.cu:2471-2500
+        if (info) *info = vox_solve_info{};
+        return st_pressure_wc(c);
+    }
     VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));    // functions.cu:2503
     VOX_TRY(ppe_assemble(c));
     int rc = ppe_solve(c, info);
@@ -509,18 +522,23 @@ int vox_step(vox_ctx* c, int nsteps)
     VOX_TRY(need_init(c));
     for (int s = 0; s < nsteps; ++s) {
         if (c->p.dt_mode == 1) VOX_TRY(vox_dtcalc(c));      // main.cu:1030
-        VOX_TRY(st_pnd(c, 0, 1));                           // 1036-1041 (previous list)
+        const bool wc = (c->p.method == 3);
+        if (!wc) VOX_TRY(st_pnd(c, 0, 1));                  // 1036-1041 (previous list)
         VOX_TRY(build_neighbors(c));                        // 1050-1055
         if (c->p.turb) VOX_TRY(st_sps(c));                  // 1066-1071
         VOX_TRY(st_vfraction(c));                           // 1084-1087
         VOX_TRY(st_viscosity(c));                           // 1096-1099
         VOX_TRY(st_prediction(c));                          // 1108-1111
         VOX_TRY(st_pnd(c, 1, 1));                           // 1121-1124
-        VOX_TRY(st_bcon(c));                                // 1133-1138
-        VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp)); // 1147-1151 -> 2503
-        VOX_TRY(ppe_assemble(c));
-        VOX_TRY(ppe_solve(c, nullptr));
-        VOX_TRY(st_clamp_pressure(c));
+        if (wc) {
+            VOX_TRY(st_pressure_wc(c));                     // 1149-1151 -> 2471-2500
+        } else {
+            VOX_TRY(st_bcon(c));                            // 1133-1138
+            VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp)); // 1147-1151 -> 2503
+            VOX_TRY(ppe_assemble(c));
+            VOX_TRY(ppe_solve(c, nullptr));
+            VOX_TRY(st_clamp_pressure(c));
+        }
         VOX_TRY(st_phat(c));                                // 1267-1270
         VOX_TRY(st_pressgrad(c));                           // 1279-1282
         VOX_TRY(st_euler(c));                               // 1291-1294

@@ -358,6 +358,7 @@ int st_n0(vox_ctx* c);
 
 // PPE (vox_ppe.hip)
 int ppe_assemble(vox_ctx* c);
+int st_pressure_wc(vox_ctx* c);   // PRESSURECALC method 3
 int ppe_solve(vox_ctx* c, vox_solve_info* info);
 int ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32_t* cols,
                double* vals, double* rhs, int32_t* ids);

@@ -34,8 +34,9 @@ __global__ void k_umap(int n, const int32_t* __restrict__ bcon, int32_t* __restr
 }
 
 struct AsmArgs {
-    int n, dim, ktype, acap;
+    int n, dim, ktype, acap, matopt;
     double re, c0, n0l, rho1, dshift, inv_dt_dt, n0s;
+    double hl;                             // matopt: 3 re (2D) or 2 re (3D)
     double inv_re, inv_n0l, inv_rho1;     // correctly rounded reciprocals
     ListView L;
     const double *x, *y, *z, *pnd;
@@ -128,9 +129,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             const double d = sqrt(dx * dx + dy * dy + dz * dz);
             // jj == i: self, or padding past the row end
             const bool use = (jj[u] != i) && (bj != -1) && !(fabs(d) <= 1.0e-15);
-            // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by)
-            const double val = div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
-                                      a.rho1, a.inv_rho1);
+            // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by);
+            // matopt: hl / d^3 / n0l / rho1 (functions.cu:903-913)
+            const double val = a.matopt
+                ? div_by(div_by(a.hl / (d * d * d), a.n0l, a.inv_n0l), a.rho1, a.inv_rho1)
+                : div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
+                         a.rho1, a.inv_rho1);
             if (use && bj == 0 && val != 0.0) {
                 rc[k & 15] = rj.aux;
                 rv[k & 15] = -val;
@@ -202,6 +206,88 @@ __global__ void k_aval32(int nu, int cap, const int32_t* __restrict__ rowlen,
     for (; k < len; ++k) aval32[ob + 64L * k] = (float)aval[ob + 64L * k];
 }
 
+// ---- higher-order source with error compensation (srcopt, functions.cu:928-988)
+struct SrcArgs {
+    int n, dim;
+    double re, hs_coef, n0s, dt;
+    ListView L;
+    const double *x, *y, *z, *us, *vs, *ws, *nstar;
+    const int32_t *bcon, *ptype;
+    double* hs;
+};
+
+// The divergence part of row i (bcon == 0): hs_i = (-1.25 / (n0 dt)) * sum_j
+// (re / d^3) ((x_i - x_j) du + (y_i - y_j) dv [+ (z_i - z_j) dw]) over list L,
+// du = u*_i - u*_j, skipping dummies, self and coincident particles.
+__global__ __launch_bounds__(kBlock) void k_src_hs(SrcArgs a)
+{
+    const int i = xcd_tid();
+    if (i >= a.n) return;
+    if (a.ptype[i] == -1 || a.bcon[i] != 0) return;
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    const double xi = a.x[i], yi = a.y[i], zi = (a.dim == 3) ? a.z[i] : 0.0;
+    const double ui = a.us[i], vi = a.vs[i], wi = (a.dim == 3) ? a.ws[i] : 0.0;
+    double sum = 0.0;
+    const int cnt = a.L.cnt[i];
+    for (int l = 0; l < cnt; ++l) {
+        const int j = row[sl_step(l)];
+        if (a.ptype[j] <= -1 || i == j) continue;
+        const double dx = a.x[j] - xi, dy = a.y[j] - yi;
+        double d;
+        if (a.dim == 3) {
+            const double dz = a.z[j] - zi;
+            d = sqrt(dx * dx + dy * dy + dz * dz);
+        } else {
+            d = sqrt(dx * dx + dy * dy);
+        }
+        if (fabs(d) <= 1.0e-15) continue;
+        const double du = ui - a.us[j], dv = vi - a.vs[j];
+        if (a.dim == 3) {
+            const double dw = wi - a.ws[j];
+            sum += (a.re / (d * d * d)) * ((xi - a.x[j]) * du + (yi - a.y[j]) * dv + (zi - a.z[j]) * dw);
+        } else {
+            sum += (a.re / (d * d * d)) * ((xi - a.x[j]) * du + (yi - a.y[j]) * dv);
+        }
+    }
+    a.hs[i] = a.hs_coef * sum;
+}
+
+// The error-compensating term reads the source just written for particle
+// i - 1 (the reference's sequential loop), so rows are chained in particle
+// order. A chain starts where particle i - 1 is not an unknown (its source is
+// 0); one thread walks each chain. source[] holds 0 for every non-unknown.
+__global__ __launch_bounds__(kBlock) void k_src_chain(int n, const int32_t* __restrict__ bcon,
+                                                      const int32_t* __restrict__ ptype,
+                                                      const double* __restrict__ hs,
+                                                      const double* __restrict__ nstar, double n0s,
+                                                      double dt, double* source)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto unknown = [&](int k) { return bcon[k] == 0 && ptype[k] != -1; };
+    if (!unknown(i) || (i > 0 && unknown(i - 1))) return;
+    double prev = (i > 0) ? source[i - 1] : 0.0;
+    for (int k = i; k < n && unknown(k); ++k) {
+        double ecs = 0.0;
+        if (k > 0) {
+            const double alpha = (nstar[k] - n0s) / n0s;
+            const double beta = (dt / n0s) * (prev / (-1.0 / (n0s * dt)));
+            ecs = fabs(alpha) * (beta / dt) + fabs(beta) * (alpha / dt);
+        }
+        double s = hs[k] + ecs;
+        if (fabs(s) < 0.99) s = 0;
+        source[k] = s;
+        prev = s;
+    }
+}
+
+__global__ void k_rhs_from_source(int nu, const int32_t* __restrict__ urow, const double* __restrict__ source,
+                                  double* rhs)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nu) rhs[r] = source[urow[r]];
+}
+
 int ppe_assemble(vox_ctx* c)
 {
     ProfScope ps(c, KC_ASSEMBLE);
@@ -230,6 +316,8 @@ int ppe_assemble(vox_ctx* c)
     AsmArgs a;
     a.n = n; a.dim = p.dim; a.ktype = p.ktype; a.re = p.re_l;
     a.c0 = 2.0 * p.dim / p.lambda;
+    a.matopt = p.matopt;
+    a.hl = (p.dim == 2) ? 3.0 * p.re_l : 2.0 * p.re_l;
     a.n0l = c->n0l; a.rho1 = p.rho1;
     a.inv_re = 1.0 / a.re; a.inv_n0l = 1.0 / a.n0l; a.inv_rho1 = 1.0 / a.rho1;
     a.dshift = (1.00e-7) / p.dt / p.dt;
@@ -255,8 +343,58 @@ int ppe_assemble(vox_ctx* c)
                                                                    c->aval);
         VOX_HIP(hipGetLastError());
     }
+    if (p.srcopt) {
+        SrcArgs s;
+        s.n = n; s.dim = p.dim; s.re = p.re_l; s.n0s = c->n0s; s.dt = p.dt;
+        s.hs_coef = -1.25 / (c->n0l * p.dt);
+        s.L = view(c, VOX_LIST_L);
+        s.x = c->f[VOX_F_X]; s.y = c->f[VOX_F_Y]; s.z = c->f[VOX_F_Z];
+        s.us = c->f[VOX_F_USTAR]; s.vs = c->f[VOX_F_VSTAR]; s.ws = c->f[VOX_F_WSTAR];
+        s.nstar = c->f[VOX_F_NSTAR]; s.bcon = c->bcon; s.ptype = c->ptype;
+        s.hs = c->scratch_d;
+        k_src_hs<<<grid_for(n), kBlock, 0, c->stream>>>(s);
+        k_src_chain<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->ptype, c->scratch_d,
+                                                           c->f[VOX_F_NSTAR], c->n0s, p.dt,
+                                                           c->f[VOX_F_SOURCE]);
+        if (c->nu > 0)
+            k_rhs_from_source<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->urow,
+                                                                         c->f[VOX_F_SOURCE], c->rhs);
+        VOX_HIP(hipGetLastError());
+    }
     c->aval32_ok = false;
     if (mixed) VOX_TRY(ppe_make_aval32(c));
+    return VOX_OK;
+}
+
+// PRESSURECALC, method 3 (functions.cu:2471-2500): equation of state on n*
+// for every non-dummy particle; dummies keep their pressure.
+__global__ void k_pressure_wc(int gp, int n, const int32_t* __restrict__ ptype, const double* __restrict__ C,
+                              const double* __restrict__ nstar, double c01, double c02, double rho1,
+                              double rho2, double beta, double n0, double pmin, double pmax, double* pnew)
+{
+    const int i = gp + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double c0 = (ptype[i] == 2) ? c02 : c01;
+    const double rho = rho2 * C[i] + rho1 * (1 - C[i]);
+    double p;
+    if (nstar[i] < beta * n0) p = 0.0;
+    else p = (c0 * c0 * rho / 7.0) * (pow(nstar[i] / n0, 7.0) - 1);
+    if (p < pmin) p = pmin;
+    if (p > pmax) p = pmax;
+    pnew[i] = p;
+}
+
+int st_pressure_wc(vox_ctx* c)
+{
+    ProfScope ps(c, KC_MISC);
+    const vox_params& p = c->p;
+    const int m = c->tp - c->gp;
+    if (m <= 0) return VOX_OK;
+    k_pressure_wc<<<grid_for(m), kBlock, 0, c->stream>>>(c->gp, c->tp, c->ptype, c->f[VOX_F_C],
+                                                         c->f[VOX_F_NSTAR], p.c01, p.c02, p.rho1, p.rho2,
+                                                         p.wc_beta, c->n0s, p.pmin, p.pmax,
+                                                         c->f[VOX_F_PNEW]);
+    VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
 

@@ -52,6 +52,10 @@ def default_params(dim, dl):
     p.warm_start = 0
     p.list_mode = 0
     p.lpc_cap_pct = 200
+    p.method = 1
+    p.matopt = 0
+    p.srcopt = 0
+    p.wc_beta = 0.92
     return p
 
 
@@ -126,3 +130,22 @@ def ensemble_params(case, dt=1e-4, tol=1e-8, box=None):
         return set_box(p, case)
     p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = (float(v) for v in box)
     return p
+
+
+def wc_params(case, dt=None):
+    """Weakly compressible MPS (PRESSURECALC method 3, main.cu:249-291 with
+    `comp`): cubic kernel (KTYPE 6), relaxp 0.5, collision coefficient 0.5,
+    equation-of-state free-surface ratio 0.92, sound speeds 20 m/s; one radius
+    4.0 l0 in 2D and 3.2 l0 in 3D (main.cu:252-259)."""
+    p = default_params(case.dim, case.l0)
+    k = 4.0 if case.dim == 2 else 3.2
+    p.re_s = p.re_l = k * case.l0
+    p.lambda_ = 0.16832 * (p.re_l * p.re_l)
+    p.ktype = 6
+    p.method = 3
+    p.relaxp = 0.5
+    p.cc = 0.5
+    p.wc_beta = 0.92
+    p.gy = -9.81
+    p.dt = dt if dt is not None else 1e-4
+    return set_box(p, case)
