@@ -171,8 +171,15 @@ __global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
     for (int g = 0; (g << 3) < len; ++g) {
         int32_t cc[8];
         double vv[8];
-        load8_nt(tcol + rb + ((long)g << 9), cc);
-        load8_nt(tval + rb + ((long)g << 9), vv);
+        {
+            const v4i* cp = reinterpret_cast<const v4i*>(tcol + rb + ((long)g << 9));
+            const v4i c0 = cp[0], c1 = cp[1];
+            cc[0] = c0.x; cc[1] = c0.y; cc[2] = c0.z; cc[3] = c0.w;
+            cc[4] = c1.x; cc[5] = c1.y; cc[6] = c1.z; cc[7] = c1.w;
+            const v2d* vp = reinterpret_cast<const v2d*>(tval + rb + ((long)g << 9));
+#pragma unroll
+            for (int h = 0; h < 4; ++h) { const v2d t = vp[h]; vv[2 * h] = t.x; vv[2 * h + 1] = t.y; }
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int kk = (g << 3) + u;
