@@ -716,6 +716,173 @@ static void spmv(const orc_ctx* c, const double* xin, double* s)
     }
 }
 
+/* ---- INCDECOM + CGM, the reference's IC(0)-CG: functions.cu:1139-1200, 506-716.
+ * Sequential, in the reference's loop order over list-L slots; a 3D row lists
+ * the particle itself, so its slot takes part as the loops let it (the
+ * triangular solves then read q[i] of the previous application there). */
+static void ic_solve(const orc_ctx* c, const double* ic, const double* icd, const double* r,
+                     double* aux, double* q)
+{
+    const orc_list* L = LL(c);
+    const int tp = c->tp;
+    memcpy(aux, r, sizeof(double) * tp);
+    for (int i = 0; i < tp; ++i) {                                      /* 531-546 */
+        if (c->bcon[i] != 0) continue;
+        const int32_t* row = ROW(L, i);
+        const double* a = ic + (size_t)i * L->cap;
+        for (int l = 0; l < L->cnt[i]; ++l) {
+            int j = row[l];
+            if (j > i) continue;
+            if (c->bcon[j] != 0) continue;
+            aux[i] = aux[i] - a[l] * q[j];
+        }
+        q[i] = aux[i] / icd[i];
+    }
+    memcpy(aux, q, sizeof(double) * tp);
+    for (int i = tp - 1; i >= 0; --i) {                                 /* 548-563 */
+        if (c->bcon[i] != 0) continue;
+        const int32_t* row = ROW(L, i);
+        const double* a = ic + (size_t)i * L->cap;
+        for (int l = 0; l < L->cnt[i]; ++l) {
+            int j = row[l];
+            if (j < i) continue;
+            if (c->bcon[j] != 0) continue;
+            aux[i] = aux[i] - a[l] * q[j];
+        }
+        q[i] = aux[i] / icd[i];
+    }
+}
+
+static void a_times(const orc_ctx* c, const double* xin, double* s)
+{
+    const orc_list* L = LL(c);
+    for (int i = 0; i < c->tp; ++i) {                                   /* 518-528, 578-588 */
+        if (c->bcon[i] != 0) continue;
+        s[i] = c->adiag[i] * xin[i];
+        const int32_t* row = ROW(L, i);
+        const double* a = c->aval + (size_t)i * L->cap;
+        for (int l = 0; l < L->cnt[i]; ++l) {
+            int j = row[l];
+            if (c->bcon[j] == -1) continue;
+            s[i] = s[i] + a[l] * xin[j];
+        }
+    }
+}
+
+int orc_ppe_solve_iccg(orc_ctx* c, int imin, int imax, double eps, vox_solve_info* info)
+{
+    if (!c || !c->aval) return VOX_ERR_STATE;
+    const orc_list* L = LL(c);
+    const int tp = c->tp, cap = L->cap;
+    const double dt = c->p.dt;
+    double* x = c->f[VOX_F_PNEW];
+    const double* b = c->f[VOX_F_SOURCE];
+    double* ic = (double*)calloc((size_t)tp * cap, sizeof(double));
+    double* icd = (double*)calloc((size_t)tp, sizeof(double));
+    double* r = (double*)calloc((size_t)tp, sizeof(double));
+    double* p = (double*)calloc((size_t)tp, sizeof(double));
+    double* q = (double*)calloc((size_t)tp, sizeof(double));
+    double* s = (double*)calloc((size_t)tp, sizeof(double));
+    double* aux = (double*)calloc((size_t)tp, sizeof(double));
+    if (!ic || !icd || !r || !p || !q || !s || !aux) {
+        free(ic); free(icd); free(r); free(p); free(q); free(s); free(aux);
+        return VOX_ERR_ARG;
+    }
+    /* INCDECOM, 1144-1197 */
+    for (int i = 0; i < tp; ++i) {
+        if (c->bcon[i] != 0) continue;
+        const int32_t* ri = ROW(L, i);
+        double* ici = ic + (size_t)i * cap;
+        const double* ai = c->aval + (size_t)i * cap;
+        double sum = c->adiag[i];
+        for (int l = 0; l < L->cnt[i]; ++l) {
+            int j = ri[l];
+            if (j > i) continue;
+            if (c->bcon[j] != 0) continue;
+            sum = sum - ici[l] * ici[l];
+        }
+        icd[i] = sqrt(sum);
+        for (int l = 0; l < L->cnt[i]; ++l) {
+            int j = ri[l];
+            if (j < i) continue;
+            if (c->bcon[j] != 0) continue;
+            sum = ai[l];
+            const int32_t* rj = ROW(L, j);
+            double* icj = ic + (size_t)j * cap;
+            for (int mj = 0; mj < L->cnt[j]; ++mj) {
+                int kj = rj[mj];
+                if (kj >= i) continue;
+                if (c->bcon[kj] != 0) continue;
+                for (int mi = 0; mi < L->cnt[i]; ++mi) {
+                    if (ri[mi] == kj) {
+                        sum = sum - ici[mi] * icj[mj];
+                        break;
+                    }
+                }
+            }
+            ici[l] = sum / icd[i];
+            for (int mj = 0; mj < L->cnt[j]; ++mj) {
+                if (rj[mj] == i) {
+                    icj[mj] = ici[l];
+                    break;
+                }
+            }
+        }
+    }
+    /* CGM, 506-716, from x = 0 (pnew was cleared by PRESSURECALC) */
+    memset(x, 0, sizeof(double) * tp);
+    a_times(c, x, s);
+    for (int i = 0; i < tp; ++i) r[i] = b[i] - s[i];
+    ic_solve(c, ic, icd, r, aux, q);
+    memcpy(p, q, sizeof(double) * tp);
+    double rqo = 0.0;
+    for (int i = 0; i < tp; ++i)
+        if (c->bcon[i] == 0) rqo += r[i] * q[i];
+    int it = 0, conv = 0;
+    for (int k = 0; k < imax; ++k) {
+        a_times(c, p, s);
+        double ps = 0.0;
+        for (int i = 0; i < tp; ++i)
+            if (c->bcon[i] == 0) ps += p[i] * s[i];
+        double aa = rqo / ps;
+        for (int i = 0; i < tp; ++i) x[i] = x[i] + aa * p[i];
+        for (int i = 0; i < tp; ++i) r[i] = r[i] - aa * s[i];
+        ic_solve(c, ic, icd, r, aux, q);
+        double rqn = 0.0;
+        for (int i = 0; i < tp; ++i)
+            if (c->bcon[i] == 0) rqn += r[i] * q[i];
+        double bb = rqn / rqo;
+        rqo = rqn;
+        for (int i = 0; i < tp; ++i) p[i] = q[i] + bb * p[i];
+        int nbad = 0;                                                   /* 684-699 */
+        for (int i = 0; i < tp; ++i) {
+            if (c->bcon[i] != 0) continue;
+            double err1 = (r[i] > 0) ? r[i] * dt * dt : -r[i] * dt * dt;
+            if (err1 > eps) nbad++;
+        }
+        it = k + 1;
+        if (nbad == 0 && k >= imin) { conv = 1; break; }
+    }
+    if (info) {
+        double rr = 0.0, bb2 = 0.0;
+        a_times(c, x, s);
+        for (int i = 0; i < tp; ++i)
+            if (c->bcon[i] == 0) {
+                double e = b[i] - s[i];
+                rr += e * e;
+                bb2 += b[i] * b[i];
+            }
+        memset(info, 0, sizeof(*info));
+        info->iterations = it;
+        info->converged = conv;
+        info->unknowns = c->nu;
+        info->relres = bb2 > 0 ? sqrt(rr / bb2) : 0.0;
+        info->bnorm = sqrt(bb2);
+    }
+    free(ic); free(icd); free(r); free(p); free(q); free(s); free(aux);
+    return VOX_OK;
+}
+
 /* Jacobi-preconditioned CG over the bcon == 0 rows, zero start (2503). */
 int orc_ppe_solve(orc_ctx* c, vox_solve_info* info)
 {

@@ -40,6 +40,11 @@ int  orc_prediction(orc_ctx* ctx);
 int  orc_bcon(orc_ctx* ctx);
 int  orc_ppe_assemble(orc_ctx* ctx);
 int  orc_ppe_solve(orc_ctx* ctx, vox_solve_info* info);
+/* The reference's own solver on the assembled system: INCDECOM (IC(0) in the
+ * list-slot layout, functions.cu:1139-1200) and CGM (IC-preconditioned CG from
+ * a zero start, stop when max |r_i| dt^2 <= eps after at least imin
+ * iterations, at most imax; functions.cu:506-716). Writes pnew (no clamp). */
+int  orc_ppe_solve_iccg(orc_ctx* ctx, int imin, int imax, double eps, vox_solve_info* info);
 int  orc_pressurecalc(orc_ctx* ctx, vox_solve_info* info);
 int  orc_phat(orc_ctx* ctx);
 int  orc_pressgrad(orc_ctx* ctx);

@@ -40,6 +40,7 @@ PROTOTYPES = {
     "orc_bcon": (C.c_int, [_P]),
     "orc_ppe_assemble": (C.c_int, [_P]),
     "orc_ppe_solve": (C.c_int, [_P, C.POINTER(_abi.VoxSolveInfo)]),
+    "orc_ppe_solve_iccg": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.POINTER(_abi.VoxSolveInfo)]),
     "orc_pressurecalc": (C.c_int, [_P, C.POINTER(_abi.VoxSolveInfo)]),
     "orc_phat": (C.c_int, [_P]),
     "orc_pressgrad": (C.c_int, [_P]),

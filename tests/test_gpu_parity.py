@@ -7,6 +7,8 @@ the same order without contraction). Solver outputs agree to the solver
 tolerance: |p_gpu - p_cpu| <= 1e-6 * max|p| for cg_tol 1e-10. Trajectories
 agree to 1e-6 relative over the stated horizon.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -187,6 +189,14 @@ def test_stages_bit_exact(hip, oracle, name, case, p):
     assert abs(ig.iterations - ic.iterations) <= 1
     pg, pc = g.get("PNEW"), c.get("PNEW")
     assert np.max(np.abs(pg - pc)) <= 1e-6 * max(np.max(np.abs(pc)), 1.0)
+    # the reference's own solver on the same (bit-identical) system: IC(0)-CG
+    # with its stop rule (functions.cu:506-716, 1139-1200; imin 10, imax 50,
+    # eps 1e-9, main.cu:229-230)
+    info = vx.VoxSolveInfo()
+    assert oracle.lib.orc_ppe_solve_iccg(c.ctx, 10, 50, 1e-9, C.byref(info)) == 0
+    pi = c.get("PNEW")
+    assert info.converged
+    assert np.max(np.abs(pg - pi)) <= 1e-8 * max(np.max(np.abs(pi)), 1.0), f"{name} vs ICCG"
     # PRESSURECALC method 3, the equation of state on n* (functions.cu:2471-2500):
     # pow(n*/n0, 7) may differ in its last bits between the device and host
     # libraries; (n*/n0)^7 - 1 cancels near n* = n0, so the bound is absolute,
