@@ -96,3 +96,34 @@ def test_autotune_keeps_lists(hip):
     for (c0, i0), (c1, i1) in zip(before, after):
         assert np.array_equal(c0, c1) and np.array_equal(i0, i1)
     s.close()
+
+
+@pytest.mark.gpu
+def test_ensemble_long_run_learned(hip):
+    """The 64-scenario two-phase ensemble (mixed precision, SPS, Bingham fluid
+    2) for its 2000 steps (dt 1e-4 s) with the learned flexible CG and the
+    committed weights: every solve meets the tolerance (a failed learned solve
+    falls back to Jacobi-CG and is counted), and no particle goes non-finite
+    or leaves its box."""
+    from voxarmps_amd.train_lpc import load_weights
+    case, region, boxes, scen = vx.ensemble_c4()
+    p = vx.ensemble_params(case)
+    p.solver = vx._abi.SOLVER_LEARNED_FCG
+    net = vx.LearnedPreconditioner(backend=hip)
+    net.set_weights(load_weights())
+    s = vx.Simulation(p, case, backend=hip)
+    s.set_regions(region, boxes)
+    s.attach_lpc(net)
+    s.init()
+    s.autotune(reps=1)
+    iters = []
+    for k in range(20):
+        s.step(100)
+        st = s.status()
+        iters.append(st.last_iters)
+        print(f"step {100 * (k + 1)}: iterations {st.last_iters}, relres {st.last_relres:.2e}, "
+              f"fallbacks {st.lpc_fallbacks}, collisions {st.collisions}", flush=True)
+        assert st.nan_count == 0 and st.out_of_box == 0
+        assert st.last_relres <= p.cg_tol
+    s.close()
+    net.close()

@@ -189,3 +189,39 @@ def test_mixed_run_tracks_fp64(hip):
     assert b.status().last_relres <= 1e-8
     a.close()
     b.close()
+
+
+def test_fp32_stages_skip_coincident_pairs(hip):
+    """Two particles closer than one position quantum of the FP32 stages
+    (2^-32 of the box extent) coincide there (D = 0). Both FP32 stages skip
+    such a pair instead of evaluating 0 * W(0) / 0^2, so the velocities stay
+    finite (this was a NaN that ended a 2000-step mixed-precision ensemble)."""
+    case = vx.dam_break_2d_c0()
+    p = vx.dam2d_params(case)
+    p.precision = vx._abi.PREC_MIXED
+    s = vx.Simulation(p, case, backend=hip)
+    s.init()
+    s.step(5)
+    f = np.nonzero(case.ptype > 0)[0]
+    i, j = int(f[100]), int(f[101])
+    box = max(p.xmax - p.xmin, p.ymax - p.ymin)
+    for name, fld in (("X", "XSTAR"), ("Y", "YSTAR")):
+        for nm in (name, fld):
+            v = s.get(nm)
+            v[j] = v[i] + 0.25 * box * 2.0 ** -32      # a quarter of a quantum apart
+            s.set(nm, v)
+    s.PNUM()
+    s.NEIGHBOR()
+    s.PNUMSTAR()
+    s.PHATCALC()
+    s.PREDICTION()
+    for n in ("USTAR", "VSTAR", "XSTAR", "YSTAR"):
+        assert np.all(np.isfinite(s.get(n))), n
+    for n in ("XSTAR", "YSTAR"):
+        v = s.get(n)
+        v[j] = v[i] + 0.25 * box * 2.0 ** -32
+        s.set(n, v)
+    s.PRESSGRAD()
+    for n in ("UNEW", "VNEW"):
+        assert np.all(np.isfinite(s.get(n))), n
+    s.close()

@@ -72,6 +72,7 @@ enum DevFlag {
     FLAG_CG_ITERS,
     FLAG_CG_CONV,
     FLAG_CG_BREAK,
+    FLAG_DOMAIN_FIRST,    // smallest index of a particle outside the grid (atomicMin)
     FLAG_COUNT
 };
 

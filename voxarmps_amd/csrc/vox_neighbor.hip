@@ -195,6 +195,7 @@ __global__ void k_cell_keys(Grid g, int n, const double* __restrict__ x,
         key = (ic - 1) + (jc - 1) * g.ncx + (kc - 1) * g.ncx * g.ncy;   // 2826-2834, 0-based
     } else {
         atomicAdd(&flags[FLAG_DOMAIN], 1);
+        atomicMin(&flags[FLAG_DOMAIN_FIRST], i);
     }
     cell[i] = key;
     atomicAdd(&count[key], 1);
@@ -528,9 +529,23 @@ static int build_one(vox_ctx* c, int which, double re)
             VOX_HIP(hipGetLastError());
         }
         VOX_TRY(read_flags(c));
-        if (c->hflags[FLAG_DOMAIN] != 0)
+        if (c->hflags[FLAG_DOMAIN] != 0) {
+            // name the first such particle: its type, position and velocity
+            const int i = c->hflags[FLAG_DOMAIN_FIRST];
+            double v[5] = {0, 0, 0, 0, 0};
+            int32_t t = 0;
+            const int fs[5] = {VOX_F_X, VOX_F_Y, VOX_F_Z, VOX_F_U, VOX_F_V};
+            if (i >= 0 && i < c->tp) {
+                for (int k = 0; k < 5; ++k)
+                    VOX_HIP(hipMemcpy(&v[k], c->f[fs[k]] + i, sizeof(double), hipMemcpyDeviceToHost));
+                VOX_HIP(hipMemcpy(&t, c->ptype + i, sizeof(int32_t), hipMemcpyDeviceToHost));
+            }
+            char buf[256];
+            std::snprintf(buf, sizeof buf, "; first: particle %d (PTYPE %d) at (%.6g, %.6g, %.6g), velocity (%.6g, %.6g)",
+                          i, t, v[0], v[1], v[2], v[3], v[4]);
             return fail(VOX_ERR_DOMAIN, "particle outside the neighbour grid box (" +
-                                            std::to_string(c->hflags[FLAG_DOMAIN]) + " particles)");
+                                            std::to_string(c->hflags[FLAG_DOMAIN]) + " particles)" + buf);
+        }
         if (c->hflags[fo] == 0) return VOX_OK;
         int need = c->hflags[fm];
         int cap = ((need + need / 4) + 7) & ~7;
@@ -581,6 +596,7 @@ int autotune_lists(vox_ctx* c, int reps, int32_t* modes)
 int build_neighbors(vox_ctx* c)
 {
     VOX_TRY(fill_i32(c, c->dflags + FLAG_DOMAIN, 0, 1));
+    VOX_TRY(fill_i32(c, c->dflags + FLAG_DOMAIN_FIRST, 0x7fffffff, 1));
     VOX_TRY(build_one(c, 0, c->p.re_s));
     if (c->two_lists) VOX_TRY(build_one(c, 1, c->p.re_l));
     c->st.max_count_s = c->hflags[FLAG_MAXCNT_S];

@@ -244,7 +244,10 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad32(Grad32Args a)
             const float wv = wfun32(D, a.ktype, a.re);
             const float dp = (a.kh == 1) ? (rj.pn + me.pn - (me.ph + rj.ph)) : (rj.pn - me.ph);
             const float g = wv / D / D;
-            const bool use = jj[u] != i;
+            // D == 0: two particles within one position quantum (2^-32 of the
+            // box) coincide here; FP64 would see a finite, tiny D. The pair is
+            // skipped, as the predictor does, instead of producing 0 * inf.
+            const bool use = (jj[u] != i) && (D > 0.0f);
             s1 = use ? s1 + dp * dx * g : s1;
             s2 = use ? s2 + dp * dy * g : s2;
             s3 = use ? s3 + dp * dz * g : s3;
