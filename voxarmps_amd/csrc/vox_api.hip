@@ -318,6 +318,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         return bail(rc);
     if ((rc = dalloc(&c->coll_cnt, n)) || (rc = dalloc(&c->coll_off, n + 1)) ||
         (rc = dalloc(&c->coll_stamp, n)) || (rc = dalloc(&c->coll_root, n)) || (rc = dalloc(&c->coll_logidx, n)) ||
+        (rc = dalloc(&c->coll_act, n)) || (rc = dalloc(&c->coll_cnt2, n)) || (rc = dalloc(&c->coll_off2, n + 1)) ||
         (rc = dalloc(&c->coll_logn, 1)))
         return bail(rc);
     if ((rc = dalloc(&c->partials, kPartialSlots * (size_t)kMaxPartials)) || (rc = dalloc(&c->dscal, SC_COUNT)) ||
@@ -351,6 +352,7 @@ int vox_destroy(vox_ctx* c)
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
     dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
+    dfree(c->coll_act); dfree(c->coll_cnt2); dfree(c->coll_off2); dfree(c->coll_i2); dfree(c->coll_j2);
     dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->region); dfree(c->rbox); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
     dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
     lpc_free_ctx(c);

@@ -67,6 +67,7 @@ enum DevFlag {
     FLAG_COLL_CAND,       // candidate pairs of the exact pass
     FLAG_COLL_VIOLATION,  // margin of the candidate screen exceeded
     FLAG_COLL_BIG,        // a candidate component exceeds the per-thread replay bound
+    FLAG_COLL_OPEN,       // the replay moved a particle outside the active set
     FLAG_NU,              // number of PPE unknowns
     FLAG_CG_DONE,         // CG finished (converged, breakdown or cap)
     FLAG_CG_ITERS,
@@ -263,6 +264,11 @@ struct vox_ctx {
     int32_t* coll_j = nullptr;
     int32_t* coll_stamp = nullptr; // tp
     int32_t* coll_root = nullptr;  // tp: smallest particle of each candidate component
+    int32_t* coll_act = nullptr;   // tp: particle in the active set of the replay
+    int32_t* coll_cnt2 = nullptr;  // tp: active candidates of each particle
+    int32_t* coll_off2 = nullptr;  // tp + 1
+    int32_t* coll_i2 = nullptr;    // active candidates, reference order
+    int32_t* coll_j2 = nullptr;
 
     // batched scenarios: region of each particle and 6 box bounds per region
     int32_t* region = nullptr;

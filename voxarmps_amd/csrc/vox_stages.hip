@@ -778,6 +778,35 @@ struct CollArgs {
     int32_t epoch;
 };
 
+// The reference loop body's test and impulse for the pair (i, j)
+// (functions.cu:1540-1575) on the current state: returns 0 if no impulse
+// applies, else fills the impulse (um, vm, wm) and the masses.
+__device__ __forceinline__ int coll_eval(const CollArgs& a, int i, int j, double& m1, double& m2,
+                                         double& um, double& vm, double& wm)
+{
+    const int dim = a.dim;
+    const int ti = a.pt[i], tj = a.pt[j];
+    m1 = (ti == 1) ? a.rho1 : a.rho2;
+    m2 = (tj == 1) ? a.rho1 : a.rho2;
+    double dx = a.x[j] - a.x[i], dy = a.y[j] - a.y[i], dz = 0.0, d;
+    if (dim == 3) { dz = a.z[j] - a.z[i]; d = sqrt(dx * dx + dy * dy + dz * dz); }
+    else d = sqrt(dx * dx + dy * dy);
+    if (!(d < a.coll)) return 0;
+    double ug = (m1 * a.u[i] + m2 * a.u[j]) / (m1 + m2);
+    double vg = (m1 * a.v[i] + m2 * a.v[j]) / (m1 + m2);
+    double wg = 0, wr = 0;
+    wm = 0;
+    if (dim == 3) wg = (m1 * a.w[i] + m2 * a.w[j]) / (m1 + m2);
+    double ur = m1 * (a.u[i] - ug);
+    double vr = m1 * (a.v[i] - vg);
+    if (dim == 3) wr = m1 * (a.w[i] - wg);
+    double vabs = (dim == 3) ? (ur * dx + vr * dy + wr * dz) / d : (ur * dx + vr * dy) / d;
+    um = (1.0 + a.cc) * vabs * dx / d;
+    vm = (1.0 + a.cc) * vabs * dy / d;
+    if (dim == 3) wm = (1.0 + a.cc) * vabs * dz / d;
+    return (vabs > 0) ? 1 : 0;
+}
+
 // Applies candidate k exactly as the reference loop body (functions.cu:1540-1595)
 // with the live positions; logs each particle the first time it moves (for the
 // undo) and accumulates its path length. Returns 1 if an impulse was applied.
@@ -786,24 +815,8 @@ __device__ __forceinline__ int coll_apply(const CollArgs& a, int k)
     const int dim = a.dim;
     const int i = a.ci[k], j = a.cj[k];
     const int ti = a.pt[i], tj = a.pt[j];
-    const double m1 = (ti == 1) ? a.rho1 : a.rho2;
-    const double m2 = (tj == 1) ? a.rho1 : a.rho2;
-    double dx = a.x[j] - a.x[i], dy = a.y[j] - a.y[i], dz = 0.0, d;
-    if (dim == 3) { dz = a.z[j] - a.z[i]; d = sqrt(dx * dx + dy * dy + dz * dz); }
-    else d = sqrt(dx * dx + dy * dy);
-    if (!(d < a.coll)) return 0;
-    double ug = (m1 * a.u[i] + m2 * a.u[j]) / (m1 + m2);
-    double vg = (m1 * a.v[i] + m2 * a.v[j]) / (m1 + m2);
-    double wg = 0, wr = 0, wm = 0;
-    if (dim == 3) wg = (m1 * a.w[i] + m2 * a.w[j]) / (m1 + m2);
-    double ur = m1 * (a.u[i] - ug);
-    double vr = m1 * (a.v[i] - vg);
-    if (dim == 3) wr = m1 * (a.w[i] - wg);
-    double vabs = (dim == 3) ? (ur * dx + vr * dy + wr * dz) / d : (ur * dx + vr * dy) / d;
-    double um = (1.0 + a.cc) * vabs * dx / d;
-    double vm = (1.0 + a.cc) * vabs * dy / d;
-    if (dim == 3) wm = (1.0 + a.cc) * vabs * dz / d;
-    if (!(vabs > 0)) return 0;
+    double m1, m2, um, vm, wm;
+    if (!coll_eval(a, i, j, m1, m2, um, vm, wm)) return 0;
     for (int side = 0; side < 2; ++side) {
         const int q = side ? j : i;
         if ((side ? tj : ti) <= 0) continue;
@@ -957,17 +970,80 @@ __global__ void k_coll_undo(int dim, const int32_t* __restrict__ logn, const dou
     if (dim == 3) { z[q] = e[5]; w[q] = e[6]; }
 }
 
+// ---- active set: the replay is restricted to the candidates that touch a
+// particle which may move. A candidate evaluates on the state its two
+// particles have when the sequential loop reaches it; if neither was moved by
+// an earlier candidate, that is the state before the pass. So every candidate
+// that hits on the state before the pass marks its movable particles active;
+// the replay runs, in reference order, exactly the candidates with an active
+// end; and if it moves a particle that is not active, that particle joins the
+// set and the replay is redone from the state before the pass. When no
+// particle outside the set moves, every skipped candidate saw the state before
+// the pass, where it does not hit: the result is the full sequential replay's.
+
+// Marks the movable particles of every candidate that hits on the current state.
+__global__ void k_coll_mark(CollArgs a, int32_t* __restrict__ act)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *a.total || k >= a.cap) return;
+    const int i = a.ci[k], j = a.cj[k];
+    double m1, m2, um, vm, wm;
+    if (!coll_eval(a, i, j, m1, m2, um, vm, wm)) return;
+    if (a.pt[i] > 0) act[i] = 1;
+    if (a.pt[j] > 0) act[j] = 1;
+}
+
+// Active candidates per particle (pass 0 counts, pass 1 writes them in order).
+__global__ __launch_bounds__(kBlock) void k_coll_filter(int n, const int32_t* __restrict__ off,
+                                                        const int32_t* __restrict__ cnt,
+                                                        const int32_t* __restrict__ cj,
+                                                        const int32_t* __restrict__ act,
+                                                        int32_t* __restrict__ cnt2,
+                                                        const int32_t* __restrict__ off2,
+                                                        int32_t* __restrict__ ci2,
+                                                        int32_t* __restrict__ cj2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const bool ap = act[p] != 0;
+    int m = 0;
+    const int s = off[p], e = s + cnt[p];
+    for (int k = s; k < e; ++k) {
+        const int q = cj[k];
+        if (!(ap || act[q] != 0)) continue;
+        if (off2) { ci2[off2[p] + m] = p; cj2[off2[p] + m] = q; }
+        ++m;
+    }
+    if (!off2) cnt2[p] = m;
+}
+
+// A particle moved by the replay that is not active: add it, flag the pass.
+__global__ void k_coll_closure(const int32_t* __restrict__ logn, const double* __restrict__ log,
+                               int32_t* __restrict__ act, int32_t* __restrict__ flags)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *logn) return;
+    const int q = (int)log[8L * k];
+    if (act[q] == 0) {
+        act[q] = 1;
+        flags[FLAG_COLL_OPEN] = 1;
+    }
+}
+
 static int coll_reserve(vox_ctx* c, long need)
 {
     if (need <= c->coll_cap) return VOX_OK;
     long cap = need + need / 2 + 1024;
-    if (c->coll_i) (void)hipFree(c->coll_i);
-    if (c->coll_j) (void)hipFree(c->coll_j);
+    for (int32_t** q : {&c->coll_i, &c->coll_j, &c->coll_i2, &c->coll_j2}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
     if (c->coll_log) (void)hipFree(c->coll_log);
-    c->coll_i = c->coll_j = nullptr;
     c->coll_log = nullptr;
     VOX_HIP(hipMalloc(&c->coll_i, sizeof(int32_t) * cap));
     VOX_HIP(hipMalloc(&c->coll_j, sizeof(int32_t) * cap));
+    VOX_HIP(hipMalloc(&c->coll_i2, sizeof(int32_t) * cap));
+    VOX_HIP(hipMalloc(&c->coll_j2, sizeof(int32_t) * cap));
     VOX_HIP(hipMalloc(&c->coll_log, sizeof(double) * 8 * (2 * cap)));
     c->coll_cap = cap;
     return VOX_OK;
@@ -979,6 +1055,7 @@ __global__ void k_coll_reset(int32_t* __restrict__ flags, int32_t* __restrict__ 
         flags[FLAG_COLL_HITS] = 0;
         flags[FLAG_COLL_VIOLATION] = 0;
         flags[FLAG_COLL_BIG] = 0;
+        flags[FLAG_COLL_OPEN] = 0;
         *logn = 0;
     }
 }
@@ -1024,10 +1101,6 @@ int st_collision(vox_ctx* c)
         k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, c->coll_off,
                                                               nullptr, c->coll_i, c->coll_j,
                                                               c->coll_cap);
-        if (++c->coll_epoch == INT32_MAX) {
-            VOX_TRY(fill_i32(c, c->coll_stamp, 0, n));
-            c->coll_epoch = 1;
-        }
         CollArgs a;
         a.dim = p.dim; a.coll = p.coll; a.cc = p.cc; a.rho1 = p.rho1; a.rho2 = p.rho2; a.dt = p.dt;
         // Once the screen radius covers the list's own accept radius every
@@ -1039,32 +1112,59 @@ int st_collision(vox_ctx* c)
         a.x = X; a.y = Y; a.z = Z; a.u = U; a.v = V; a.w = W;
         a.stamp = c->coll_stamp; a.logidx = c->coll_logidx; a.logn = c->coll_logn;
         a.flags = c->dflags; a.log = c->coll_log; a.epoch = c->coll_epoch;
-        k_coll_reset<<<1, kWave, 0, c->stream>>>(c->dflags, c->coll_logn);
-        k_coll_roots<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->coll_off, c->coll_cnt, c->coll_j,
-                                                             c->coll_root, c->dflags);
-        k_coll_verify<<<grid_for(total), kBlock, 0, c->stream>>>(c->coll_off + n, c->coll_i,
-                                                                 c->coll_j, c->coll_root, c->dflags);
+        // active set from the state before the pass
+        VOX_TRY(fill_i32(c, c->coll_act, 0, n));
+        k_coll_mark<<<grid_for(total), kBlock, 0, c->stream>>>(a, c->coll_act);
         VOX_HIP(hipGetLastError());
-        VOX_TRY(read_flags(c));
-        if (c->hflags[FLAG_COLL_BIG] == 2) {
-            k_coll_replay<<<1, kWave, 0, c->stream>>>(a);          // every candidate, one lane
-        } else {
-            k_coll_replay_comp<<<grid_for(n), kBlock, 0, c->stream>>>(a, n, c->coll_off,
-                                                                       c->coll_cnt, c->coll_root);
-            if (c->hflags[FLAG_COLL_BIG])
-                k_coll_replay_big<<<1, kWave, 0, c->stream>>>(a, c->coll_root);
+        for (int round = 0;; ++round) {
+            // the active candidates, in reference order
+            k_coll_filter<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->coll_off, c->coll_cnt, c->coll_j,
+                                                                c->coll_act, c->coll_cnt2, nullptr,
+                                                                nullptr, nullptr);
+            VOX_TRY(scan_exclusive(c, c->coll_cnt2, c->coll_off2, n, c->coll_off2 + n));
+            k_coll_filter<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->coll_off, c->coll_cnt, c->coll_j,
+                                                                c->coll_act, c->coll_cnt2, c->coll_off2,
+                                                                c->coll_i2, c->coll_j2);
+            CollArgs r = a;
+            r.total = c->coll_off2 + n; r.ci = c->coll_i2; r.cj = c->coll_j2;
+            if (++c->coll_epoch == INT32_MAX) {
+                VOX_TRY(fill_i32(c, c->coll_stamp, 0, n));
+                c->coll_epoch = 1;
+            }
+            r.epoch = c->coll_epoch;
+            k_coll_reset<<<1, kWave, 0, c->stream>>>(c->dflags, c->coll_logn);
+            k_coll_roots<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->coll_off2, c->coll_cnt2, c->coll_j2,
+                                                                 c->coll_root, c->dflags);
+            k_coll_verify<<<grid_for(total), kBlock, 0, c->stream>>>(c->coll_off2 + n, c->coll_i2,
+                                                                     c->coll_j2, c->coll_root, c->dflags);
+            VOX_HIP(hipGetLastError());
+            VOX_TRY(read_flags(c));
+            if (c->hflags[FLAG_COLL_BIG] == 2) {
+                k_coll_replay<<<1, kWave, 0, c->stream>>>(r);          // every active candidate, one lane
+            } else {
+                k_coll_replay_comp<<<grid_for(n), kBlock, 0, c->stream>>>(r, n, c->coll_off2,
+                                                                           c->coll_cnt2, c->coll_root);
+                if (c->hflags[FLAG_COLL_BIG])
+                    k_coll_replay_big<<<1, kWave, 0, c->stream>>>(r, c->coll_root);
+            }
+            VOX_HIP(hipGetLastError());
+            // at most two log entries per candidate
+            k_coll_check<<<grid_for(2L * total), kBlock, 0, c->stream>>>(r);
+            k_coll_closure<<<grid_for(2L * total), kBlock, 0, c->stream>>>(c->coll_logn, c->coll_log,
+                                                                          c->coll_act, c->dflags);
+            VOX_HIP(hipGetLastError());
+            VOX_TRY(read_flags(c));
+            c->st.collisions = c->hflags[FLAG_COLL_HITS];
+            const bool violation = c->hflags[FLAG_COLL_VIOLATION] != 0;
+            const bool open = c->hflags[FLAG_COLL_OPEN] != 0;
+            if (!violation && !open) return VOX_OK;
+            // undo; then either widen the screen or replay with the larger set
+            k_coll_undo<<<grid_for(2L * total), kBlock, 0, c->stream>>>(p.dim, c->coll_logn, c->coll_log,
+                                                                       X, Y, Z, U, V, W);
+            VOX_HIP(hipGetLastError());
+            if (violation) break;
+            if (round > 64) return fail(VOX_ERR_SOLVER, "collision active set did not close");
         }
-        VOX_HIP(hipGetLastError());
-        // at most two log entries per candidate
-        k_coll_check<<<grid_for(2L * total), kBlock, 0, c->stream>>>(a);
-        VOX_HIP(hipGetLastError());
-        VOX_TRY(read_flags(c));
-        c->st.collisions = c->hflags[FLAG_COLL_HITS];
-        if (!c->hflags[FLAG_COLL_VIOLATION]) return VOX_OK;
-        // undo and widen the screen
-        k_coll_undo<<<grid_for(2L * total), kBlock, 0, c->stream>>>(p.dim, c->coll_logn, c->coll_log,
-                                                                   X, Y, Z, U, V, W);
-        VOX_HIP(hipGetLastError());
         c->coll_margin *= 2.0;
     }
     return fail(VOX_ERR_SOLVER, "collision screen could not bound particle motion");
