@@ -161,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
                                                           const int32_t* __restrict__ tcol,
                                                           const double* __restrict__ tval,
                                                           int32_t* __restrict__ acol,
-                                                          double* __restrict__ aval)
+                                                          double* __restrict__ aval,
+                                                          float* __restrict__ aval32)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nu) return;
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
             if (kk < len) {
                 acol[ob + 64L * kk] = cc[u];
                 aval[ob + 64L * kk] = vv[u];
+                if (aval32) aval32[ob + 64L * kk] = (float)vv[u];   // mixed precision
             }
         }
     }
@@ -344,12 +346,16 @@ int ppe_assemble(vox_ctx* c)
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
+    c->aval32_ok = false;
+    if (mixed && !c->aval32)
+        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
     if (c->nu > 0) {
         k_asm_transpose<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen,
                                                                    c->tcol, c->tval, c->acol,
-                                                                   c->aval);
+                                                                   c->aval, mixed ? c->aval32 : nullptr);
         VOX_HIP(hipGetLastError());
     }
+    c->aval32_ok = mixed;
     if (p.srcopt) {
         SrcArgs s;
         s.n = n; s.dim = p.dim; s.re = p.re_l; s.n0s = c->n0s; s.dt = p.dt;
@@ -368,8 +374,6 @@ int ppe_assemble(vox_ctx* c)
                                                                          c->f[VOX_F_SOURCE], c->rhs);
         VOX_HIP(hipGetLastError());
     }
-    c->aval32_ok = false;
-    if (mixed) VOX_TRY(ppe_make_aval32(c));
     return VOX_OK;
 }
 
