@@ -213,6 +213,8 @@ def main():
                     help="PPE solver of the step loop: Jacobi-PCG, or flexible CG with the "
                          "learned preconditioner (Jacobi-CG fallback)")
     ap.add_argument("--lpc-weights", default=None, help="weights of the learned preconditioner")
+    ap.add_argument("--no-ppe-compare", action="store_true",
+                    help="skip the Jacobi-CG / learned flexible CG comparison on the last system")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
     ap.add_argument("--precision-report", type=str, default="", metavar="STEPS",
@@ -306,7 +308,7 @@ def main():
         n = C.c_int64(0)
         be.lib.vox_prof_get(sim.ctx, k.encode(), C.byref(t), C.byref(n))
         per_kernel[k] = round(t.value / max(args.steps, 1), 4)
-    ppe = ppe_compare(sim, p, net) if (net is not None and not ensemble) else None
+    ppe = ppe_compare(sim, p, net) if (net is not None and not ensemble and not args.no_ppe_compare) else None
     fallbacks = st.lpc_fallbacks
     sim.close()
 
