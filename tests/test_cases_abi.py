@@ -95,6 +95,38 @@ def test_create_rejects_bad_arguments():
     assert rc != 0
 
 
+@pytest.mark.parametrize("field,value,code,text", [
+    ("method", 2, _abi.VOX_ERR_NOLIB, b"method"),          # Method 2 (M-MPS) is not built
+    ("matopt", 2, _abi.VOX_ERR_ARG, b"matopt"),
+    ("srcopt", -1, _abi.VOX_ERR_ARG, b"srcopt"),
+    ("solver", 5, _abi.VOX_ERR_NOLIB, b"solver"),
+    ("lpc_cap_pct", 0, _abi.VOX_ERR_ARG, b"lpc_cap_pct"),
+    ("list_mode", 3, _abi.VOX_ERR_ARG, b"list_mode"),
+    ("ir_maxouter", 0, _abi.VOX_ERR_ARG, b"ir_maxouter"),
+])
+def test_create_validates_options(field, value, code, text):
+    """Options outside their documented values are refused before any device
+    work (so this runs without a GPU)."""
+    be = vx.hip_backend()
+    p = vx.default_params(2, 0.01)
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 1.0, 0.0, 1.0
+    setattr(p, field, value)
+    ctx = C.c_void_p()
+    rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 3, 1, 1, 1)
+    assert rc == code and text in be.lib.vox_last_error()
+
+
+def test_lpc_entry_points_check_arguments():
+    """The learned-preconditioner entry points refuse null handles and
+    arguments without touching a device."""
+    be = vx.hip_backend()
+    assert be.lib.vox_lpc_create(None, 0) == _abi.VOX_ERR_ARG
+    assert be.lib.vox_lpc_get_weights(None, None) == _abi.VOX_ERR_ARG
+    assert be.lib.vox_lpc_train(None, 1, 1e-3, 0, None, None) == _abi.VOX_ERR_ARG
+    assert be.lib.vox_attach_lpc(None, None) == _abi.VOX_ERR_ARG
+    assert be.lib.vox_lpc_destroy(None) == 0
+
+
 def test_grd_round_trip_through_c_reader(tmp_path):
     be = vx.hip_backend()
     for c in (vx.jitter(vx.dam_break_2d_c0(), 0.001),
