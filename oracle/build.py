@@ -1,7 +1,10 @@
 """Build the CPU oracle (test infrastructure) as oracle/liboracle.so.
 
 gcc without -mfma and with -ffp-contract=off keeps every double operation a
-separate IEEE operation in source order, like the HIP library.
+separate IEEE operation in source order, like the HIP library. -fopenmp runs
+the per-particle loops on the host's cores (OMP_NUM_THREADS); each particle's
+sums keep their order and the reductions left in parallel loops are max and
+integer counts, so results do not depend on the thread count.
 """
 import os
 import subprocess
@@ -14,7 +17,7 @@ LIB = os.path.join(HERE, "liboracle.so")
 def build_oracle(verbose=False):
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           "-Wall", "-o", tmp, os.path.join(HERE, "oracle.c"), "-lm"]
+           "-fopenmp", "-Wall", "-o", tmp, os.path.join(HERE, "oracle.c"), "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stderr}")

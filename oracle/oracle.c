@@ -23,6 +23,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef struct orc_list {
     double   re;
@@ -228,6 +231,7 @@ static int build_list(orc_ctx* c, int w, double re)
      * the grid edge, ascending id inside each cell. Two passes: count, fill. */
     for (int pass = 0; pass < 2; ++pass) {
         int maxc = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : maxc)
         for (int i = 0; i < tp; ++i) {
             int ic, jc, kc;
             cell_coords(p, &g, x[i], y[i], z[i], &ic, &jc, &kc);
@@ -340,6 +344,7 @@ int orc_pnum(orc_ctx* c, int loopstarted)
     const double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
     double* n = c->f[VOX_F_N];
     double mx = 0.0;                           /* MAX = new double[1]() (main.cu:376) */
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : mx)
     for (int i = 0; i < c->tp; ++i) {
         n[i] = pnd_row(c, LS(c), c->p.re_s, x, y, z, i);
         if (!loopstarted) {
@@ -351,6 +356,7 @@ int orc_pnum(orc_ctx* c, int loopstarted)
         c->n0s = mx;                                  /* main.cu:1011 */
         if (c->two_lists) {
             double ml = 0.0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : ml)
             for (int i = 0; i < c->tp; ++i) {
                 double v = pnd_row(c, LL(c), c->p.re_l, x, y, z, i);
                 if (v > ml) ml = v;
@@ -368,6 +374,7 @@ int orc_pnumstar(orc_ctx* c)
 {
     const double *x = c->f[VOX_F_XSTAR], *y = c->f[VOX_F_YSTAR], *z = c->f[VOX_F_ZSTAR];
     double* ns = c->f[VOX_F_NSTAR];
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {
         double sum = 0.0;
         const int32_t* r = ROW(LS(c), i);
@@ -438,6 +445,7 @@ int orc_sps(orc_ctx* c)
     if (!S) return VOX_ERR_ARG;
     double* neut = c->f[VOX_F_NEUT];
     double csdl = c->p.cs * c->p.dl;
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = c->gp; i < tp; ++i) {
         S[i] = strain_row(c, i, c->p.coll / 10, 1);
         double s = sqrt(2 * S[i].s11 * S[i].s11) + 2 * sqrt(2 * S[i].s12 * S[i].s12) +
@@ -450,6 +458,7 @@ int orc_sps(orc_ctx* c)
     const double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
     const orc_list* L = LS(c);
     double n0 = c->n0s;
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = c->gp; i < tp; ++i) {
         double s1 = 0, s2 = 0, s3 = 0;
         const int32_t* r = ROW(L, i);
@@ -482,6 +491,7 @@ int orc_vfraction(orc_ctx* c)
 {
     const double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
     const orc_list* L = LS(c);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {
         double s1 = 0, s2 = 0;
         const int32_t* r = ROW(L, i);
@@ -513,6 +523,7 @@ int orc_viscosity(orc_ctx* c)
         }
         return VOX_OK;
     }
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {                        /* 2087-2170 */
         if (c->ptype[i] == 1) { meu[i] = p->nu1 * p->rho1; continue; }
         strain_t S = strain_row(c, i, 0.0, 0);
@@ -550,6 +561,7 @@ int orc_prediction(orc_ctx* c)
     double lapc = p->coef2d ? 4.0 : 2.0 * dim;      /* the literal 4 of 2354-2358 */
     double grdc = p->coef2d ? 2.0 : (double)dim;    /* the literal 2.0 of 2354-2358 */
     double dt = p->dt;
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {
         if (c->ptype[i] <= 0) {
             xs[i] = x[i]; ys[i] = y[i]; us[i] = u[i]; vs[i] = v[i];
@@ -625,6 +637,7 @@ int orc_ppe_assemble(orc_ctx* c)
     int nu = 0;
     for (int i = 0; i < tp; ++i) c->uidx[i] = (c->bcon[i] == 0) ? nu++ : -1;
     c->nu = nu;
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < tp; ++i) {
         if (c->bcon[i] != 0) continue;
         double diag = 0.0;
@@ -702,6 +715,7 @@ int orc_ppe_assemble(orc_ctx* c)
 static void spmv(const orc_ctx* c, const double* xin, double* s)
 {
     const orc_list* L = LL(c);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {                     /* 577-591 */
         if (c->bcon[i] != 0) continue;
         double acc = c->adiag[i] * xin[i];
@@ -756,6 +770,7 @@ static void ic_solve(const orc_ctx* c, const double* ic, const double* icd, cons
 static void a_times(const orc_ctx* c, const double* xin, double* s)
 {
     const orc_list* L = LL(c);
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {                                   /* 518-528, 578-588 */
         if (c->bcon[i] != 0) continue;
         s[i] = c->adiag[i] * xin[i];
@@ -883,27 +898,48 @@ int orc_ppe_solve_iccg(orc_ctx* c, int imin, int imax, double eps, vox_solve_inf
     return VOX_OK;
 }
 
+/* Dot products of the CG below: ORC_NCHUNK fixed index ranges summed in
+ * parallel, each in index order, then the partial sums in chunk order. The
+ * order depends on tp only, not on the thread count. */
+#define ORC_NCHUNK 256
+static inline int chunk_lo(int tp, int k) { return (int)((long long)tp * k / ORC_NCHUNK); }
+static double sum_parts(const double* part)
+{
+    double s = 0.0;
+    for (int k = 0; k < ORC_NCHUNK; ++k) s += part[k];
+    return s;
+}
+
 /* Jacobi-preconditioned CG over the bcon == 0 rows, zero start (2503). */
 int orc_ppe_solve(orc_ctx* c, vox_solve_info* info)
 {
     int tp = c->tp;
     double* xs = c->f[VOX_F_PNEW];
     const double* b = c->f[VOX_F_SOURCE];
+    const int32_t* bc = c->bcon;
+    const double* ad = c->adiag;
     double* r = (double*)calloc((size_t)tp, sizeof(double));
     double* z = (double*)calloc((size_t)tp, sizeof(double));
     double* pv = (double*)calloc((size_t)tp, sizeof(double));
     double* q = (double*)calloc((size_t)tp, sizeof(double));
     if (!r || !z || !pv || !q) { free(r); free(z); free(pv); free(q); return VOX_ERR_ARG; }
+    double pa[ORC_NCHUNK], pb[ORC_NCHUNK];
     for (int i = 0; i < tp; ++i) xs[i] = 0.0;
-    double bb = 0.0, rz = 0.0;
-    for (int i = 0; i < tp; ++i) {
-        if (c->bcon[i] != 0) continue;
-        r[i] = b[i];
-        z[i] = r[i] / c->adiag[i];
-        pv[i] = z[i];
-        bb += b[i] * b[i];
-        rz += r[i] * z[i];
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < ORC_NCHUNK; ++k) {
+        double sb = 0.0, sr = 0.0;
+        for (int i = chunk_lo(tp, k); i < chunk_lo(tp, k + 1); ++i) {
+            if (bc[i] != 0) continue;
+            r[i] = b[i];
+            z[i] = r[i] / ad[i];
+            pv[i] = z[i];
+            sb += b[i] * b[i];
+            sr += r[i] * z[i];
+        }
+        pa[k] = sb;
+        pb[k] = sr;
     }
+    double bb = sum_parts(pa), rz = sum_parts(pb);
     double bnorm = sqrt(bb);
     int it = 0, conv = 0, rc = VOX_OK;
     double relres = 0.0;
@@ -912,28 +948,45 @@ int orc_ppe_solve(orc_ctx* c, vox_solve_info* info)
     } else {
         for (it = 1; it <= c->p.cg_maxit; ++it) {
             spmv(c, pv, q);
-            double pq = 0.0;
-            for (int i = 0; i < tp; ++i) if (c->bcon[i] == 0) pq += pv[i] * q[i];
+            #pragma omp parallel for schedule(static)
+            for (int k = 0; k < ORC_NCHUNK; ++k) {
+                double sp = 0.0;
+                for (int i = chunk_lo(tp, k); i < chunk_lo(tp, k + 1); ++i)
+                    if (bc[i] == 0) sp += pv[i] * q[i];
+                pa[k] = sp;
+            }
+            double pq = sum_parts(pa);
             if (!(pq > 0.0) || !isfinite(pq)) { rc = VOX_ERR_SOLVER; break; }
             double alpha = rz / pq;
-            double rr = 0.0;
-            for (int i = 0; i < tp; ++i) {
-                if (c->bcon[i] != 0) continue;
-                xs[i] = xs[i] + alpha * pv[i];
-                r[i] = r[i] - alpha * q[i];
-                rr += r[i] * r[i];
+            #pragma omp parallel for schedule(static)
+            for (int k = 0; k < ORC_NCHUNK; ++k) {
+                double sr = 0.0;
+                for (int i = chunk_lo(tp, k); i < chunk_lo(tp, k + 1); ++i) {
+                    if (bc[i] != 0) continue;
+                    xs[i] = xs[i] + alpha * pv[i];
+                    r[i] = r[i] - alpha * q[i];
+                    sr += r[i] * r[i];
+                }
+                pa[k] = sr;
             }
+            double rr = sum_parts(pa);
             relres = sqrt(rr) / bnorm;
             if (relres <= c->p.cg_tol) { conv = 1; break; }
-            double rzn = 0.0;
-            for (int i = 0; i < tp; ++i) {
-                if (c->bcon[i] != 0) continue;
-                z[i] = r[i] / c->adiag[i];
-                rzn += r[i] * z[i];
+            #pragma omp parallel for schedule(static)
+            for (int k = 0; k < ORC_NCHUNK; ++k) {
+                double sz = 0.0;
+                for (int i = chunk_lo(tp, k); i < chunk_lo(tp, k + 1); ++i) {
+                    if (bc[i] != 0) continue;
+                    z[i] = r[i] / ad[i];
+                    sz += r[i] * z[i];
+                }
+                pa[k] = sz;
             }
+            double rzn = sum_parts(pa);
             double beta = rzn / rz;
             rz = rzn;
-            for (int i = 0; i < tp; ++i) if (c->bcon[i] == 0) pv[i] = z[i] + beta * pv[i];
+            #pragma omp parallel for schedule(static)
+            for (int i = 0; i < tp; ++i) if (bc[i] == 0) pv[i] = z[i] + beta * pv[i];
         }
         if (it > c->p.cg_maxit) it = c->p.cg_maxit;
     }
@@ -962,6 +1015,7 @@ int orc_pressurecalc(orc_ctx* c, vox_solve_info* info)
         const double* ns = c->f[VOX_F_NSTAR];
         const double* C = c->f[VOX_F_C];
         const vox_params* p = &c->p;
+        #pragma omp parallel for schedule(static)
         for (int i = c->gp; i < c->tp; ++i) {
             const double c0 = (c->ptype[i] == 2) ? p->c02 : p->c01;
             const double rho = p->rho2 * C[i] + p->rho1 * (1 - C[i]);
@@ -977,6 +1031,7 @@ int orc_pressurecalc(orc_ctx* c, vox_solve_info* info)
     int rc = orc_ppe_assemble(c);
     if (rc) return rc;
     rc = orc_ppe_solve(c, info);
+    #pragma omp parallel for schedule(static)
     for (int i = 0; i < c->tp; ++i) {                         /* 2534-2547 */
         if (pn[i] < c->p.pmin) pn[i] = c->p.pmin;
         if (pn[i] > c->p.pmax || pn[i] * 0.0 != 0.0) pn[i] = c->p.pmax;
@@ -989,6 +1044,7 @@ int orc_phat(orc_ctx* c)
 {
     const orc_list* L = LS(c);
     const double* pn = c->f[VOX_F_PNEW];
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < c->tp; ++i) {
         double mn = 999999999999999;
         const int32_t* r = ROW(L, i);
@@ -1010,6 +1066,7 @@ int orc_pressgrad(orc_ctx* c)
     const orc_list* L = LS(c);
     int dim = p->dim;
     double grdc = p->coef2d ? 2.0 : (double)dim;    /* literal 2.0 of 339-343 */
+    #pragma omp parallel for schedule(dynamic, 256)
     for (int i = c->gp + c->wp; i < c->tp; ++i) {
         double s1 = 0, s2 = 0, s3 = 0;
         const int32_t* r = ROW(L, i);
@@ -1047,6 +1104,7 @@ int orc_euler(orc_ctx* c)
     double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
     double *u = c->f[VOX_F_UNEW], *v = c->f[VOX_F_VNEW], *w = c->f[VOX_F_WNEW];
     double DL = p->dl, CC = p->vel_cor;
+    #pragma omp parallel for schedule(static)
     for (int i = c->gp + c->wp; i < c->tp; ++i) {
         x[i] = x[i] + u[i] * p->dt;
         y[i] = y[i] + v[i] * p->dt;
@@ -1131,6 +1189,7 @@ int orc_prepdata(orc_ctx* c)
     double *u = c->f[VOX_F_U], *v = c->f[VOX_F_V], *w = c->f[VOX_F_W], *pp = c->f[VOX_F_P];
     int tp = c->tp, dim = p->dim, nanc = 0, oob = 0;
     int first = (p->test_kind == VOX_TEST_DROP) ? tp - c->fp - 1 : tp - c->fp;
+    #pragma omp parallel for schedule(static) reduction(+ : nanc, oob)
     for (int i = 0; i < tp; ++i) {
         pp[i] = c->f[VOX_F_PNEW][i];
         if (i >= first) {
@@ -1150,6 +1209,28 @@ int orc_prepdata(orc_ctx* c)
     c->st.nan_count = nanc;
     c->st.out_of_box = oob;
     return VOX_OK;
+}
+
+/* Threads the parallel loops above run on (1 without OpenMP). */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Sets that thread count (n >= 1); returns the count now in effect. */
+int orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }
 
 /* ---- DTcalculation: functions.cu:1684-1698 ---------------------------------- */

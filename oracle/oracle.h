@@ -52,6 +52,8 @@ int  orc_euler(orc_ctx* ctx);
 int  orc_collision(orc_ctx* ctx);
 int  orc_prepdata(orc_ctx* ctx);
 int  orc_dtcalc(orc_ctx* ctx);
+int  orc_threads(void);
+int  orc_set_threads(int n);
 int  orc_init(orc_ctx* ctx);
 int  orc_step(orc_ctx* ctx, int nsteps);
 int  orc_get_status(orc_ctx* ctx, vox_status* st);

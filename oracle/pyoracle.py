@@ -48,6 +48,8 @@ PROTOTYPES = {
     "orc_collision": (C.c_int, [_P]),
     "orc_prepdata": (C.c_int, [_P]),
     "orc_dtcalc": (C.c_int, [_P]),
+    "orc_threads": (C.c_int, []),
+    "orc_set_threads": (C.c_int, [C.c_int]),
     "orc_init": (C.c_int, [_P]),
     "orc_step": (C.c_int, [_P, C.c_int]),
     "orc_get_status": (C.c_int, [_P, C.POINTER(_abi.VoxStatus)]),

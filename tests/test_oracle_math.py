@@ -385,6 +385,27 @@ def test_steps_conserve_boundary_and_counts(oracle):
     assert np.all(np.isfinite(x)) and np.all(np.isfinite(y))
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_independent_of_thread_count(oracle, dim):
+    """The oracle's OpenMP loops are per particle (each sum in its own order)
+    with max / count reductions only: one thread and four give the same bits
+    over a run of steps, neighbour lists included."""
+    lib = oracle.lib
+    keep = lib.orc_threads()
+    case = vx.dam_break_2d_c0() if dim == 2 else vx.dam_break_3d_c1(8)
+    p = vx.dam2d_params(case) if dim == 2 else vx.dam3d_params(case)
+    out = []
+    try:
+        for nt in (1, 4):
+            assert lib.orc_set_threads(nt) == nt
+            with vx.Simulation(p, case, backend=oracle) as s:
+                s.init()
+                s.step(8 if dim == 2 else 3)
+                out.append([s.get(f) for f in ("X", "Y", "Z", "U", "V", "W", "PNEW", "NSTAR")])
+    finally:
+        lib.orc_set_threads(keep)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 # ---- PRESSURECALC options: matopt, srcopt, method 3 (functions.cu:901-988, 2471-2500)
 
 def test_matopt_interior_row_closed_form(oracle):
