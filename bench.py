@@ -66,8 +66,8 @@ def cpu_baseline(sample_scale=2, min_s=10.0, max_steps=20):
     """Time the CPU oracle on a bounded sample of the same workload: the 3D
     dam break at 1/sample_scale linear scale, stepped until min_s seconds have
     passed (at least 2 steps, at most max_steps). The oracle runs its
-    per-particle loops on OMP_NUM_THREADS threads (CG dot products stay
-    sequential); the thread count is reported as cores."""
+    per-particle loops and the CG's dot products on OMP_NUM_THREADS threads
+    (collision stays sequential); the thread count is reported as cores."""
     import voxarmps_amd as vx
     from oracle.pyoracle import oracle_backend
 

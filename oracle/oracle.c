@@ -1,6 +1,10 @@
 /*
  * oracle.c — CPU restatement of the reference's MPS time step (test
- * infrastructure only; see oracle.h). Sequential, FP64, 0-based arrays.
+ * infrastructure only; see oracle.h). FP64, 0-based arrays. Per-particle
+ * loops run on OpenMP threads with results independent of the thread count
+ * (each particle's sums in order; max / count reductions; the CG's dot
+ * products over fixed index ranges). Collision and the srcopt chain are
+ * sequential, as the reference defines them.
  *
  * Arithmetic is written in the reference's operation order with plain
  * IEEE double operations (no FMA: build without -mfma / with
