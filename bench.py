@@ -159,17 +159,25 @@ def precision_report(checkpoints, scale=1):
     b.close()
 
 
+PPE_SOLVERS = (("jacobi_cg", "SOLVER_JACOBI_CG", "PREC_FP64"),
+               ("learned_fcg", "SOLVER_LEARNED_FCG", "PREC_FP64"),
+               ("jacobi_mixed", "SOLVER_JACOBI_CG", "PREC_MIXED"),
+               ("learned_mixed", "SOLVER_LEARNED_FCG", "PREC_MIXED"))
+
+
 def ppe_compare(sim, p, net, reps=3):
-    """The assembled system of the last step solved cold by Jacobi-CG and by
-    the learned flexible CG: median wall ms between synchronisations,
-    iterations and relative residuals."""
+    """The assembled system of the current step solved cold, to the same
+    relative residual, by FP64 Jacobi-CG, the learned flexible CG, and both
+    inside the mixed-precision refinement (FP32 inner solves): median wall ms
+    between synchronisations, iterations, relative residuals, and each
+    variant's speed-up over FP64 Jacobi-CG."""
     import voxarmps_amd as vx
     be = sim.be
     out = {}
-    for name, solver in (("jacobi_cg", vx._abi.SOLVER_JACOBI_CG),
-                         ("learned_fcg", vx._abi.SOLVER_LEARNED_FCG)):
+    for name, solver, prec in PPE_SOLVERS:
         q = p.copy()
-        q.solver = solver
+        q.solver = getattr(vx._abi, solver)
+        q.precision = getattr(vx._abi, prec)
         sim.set_params(q)
         sim.attach_lpc(net)          # resets the baseline: the next learned solve fixes it
         sim.CGM()
@@ -191,12 +199,52 @@ def ppe_compare(sim, p, net, reps=3):
             prof[k + "_ms"] = t.value / reps
         times.sort()
         out[name] = {"ms": times[len(times) // 2], "iterations": info.iterations,
+                     "unknowns": info.unknowns,
                      "relres": info.relres, "converged": bool(info.converged),
                      "fallbacks": sim.status().lpc_fallbacks, **prof}
     sim.set_params(p)
-    out["speedup"] = out["jacobi_cg"]["ms"] / out["learned_fcg"]["ms"]
-    out["system"] = "last step's PPE, cold start, rel-res %g" % p.cg_tol
+    sim.attach_lpc(net)
+    base = out["jacobi_cg"]["ms"]
+    out["speedup"] = base / out["learned_fcg"]["ms"]
+    out["speedup_vs_jacobi_cg"] = {k: base / out[k]["ms"] for k, _, _ in PPE_SOLVERS}
+    out["system"] = "PPE of the current step, cold start, rel-res %g" % p.cg_tol
     return out
+
+
+STAGES_TO_MATRIX = ("PNUM", "NEIGHBOR", "SPS", "V_FRACTION", "VISCOSITY", "PREDICTION", "PNUMSTAR",
+                    "BCON", "MATRIX")
+
+
+def ppe_report(checkpoints, scale=1, weights=None):
+    """The FP64 run of the 3D dam break; at each checkpoint step the next
+    step's PPE is assembled (the stages up to MATRIX, as the step runs them)
+    and solved by every variant of ppe_compare. One JSON line per checkpoint.
+    The out-of-band assembly leaves the run itself unchanged: the next step
+    recomputes every field it wrote."""
+    import voxarmps_amd as vx
+    from voxarmps_amd.train_lpc import DEFAULT_WEIGHTS, load_weights
+
+    be = vx.hip_backend()
+    case = vx.dam_break_3d_c1(scale=scale)
+    p = vx.dam3d_params(case)
+    net = vx.LearnedPreconditioner(backend=be)
+    net.set_weights(load_weights(weights or DEFAULT_WEIGHTS))
+    sim = vx.Simulation(p, case, backend=be)
+    sim.init()
+    sim.attach_lpc(net)
+    done = 0
+    for cp in sorted(checkpoints):
+        while done < cp:
+            n = min(cp - done, 50)
+            sim.step(n)
+            done += n
+            print(f"# step {done}: {sim.status().last_iters} iterations", file=sys.stderr, flush=True)
+        for st in STAGES_TO_MATRIX:
+            getattr(sim, st)()
+        r = ppe_compare(sim, p, net)
+        print(json.dumps({"step": cp, "particles": case.tp, **r}), flush=True)
+    sim.close()
+    net.close()
 
 
 def main():
@@ -229,6 +277,9 @@ def main():
                     help="instead of the benchmark, run the FP64 and the mixed-precision "
                          "simulations side by side and print the pressure-field L2 and Linf "
                          "differences at the listed steps (comma separated, e.g. 100,500,2000)")
+    ap.add_argument("--ppe-report", type=str, default="", metavar="STEPS",
+                    help="instead of the benchmark, run the FP64 simulation and solve the PPE "
+                         "of each listed step (comma separated) with every solver variant")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB "
                          "(reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -241,6 +292,9 @@ def main():
         return
     if args.precision_report:
         precision_report([int(s) for s in args.precision_report.split(",")], args.scale)
+        return
+    if args.ppe_report:
+        ppe_report([int(s) for s in args.ppe_report.split(",")], args.scale, args.lpc_weights)
         return
 
     import voxarmps_amd as vx
