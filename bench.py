@@ -109,6 +109,35 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "copies": 5}))
 
 
+def stream_bandwidth(mib):
+    """Bandwidth plain torch streams reach over MIB MiB of float64 on this GPU:
+    a read-only sum and a device copy (read + write), 10 timed repetitions
+    each between HIP events."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    tc = timed(lambda: b.copy_(a))
+    ts = timed(lambda: a.sum())
+    print(json.dumps({"bytes": n * 8, "copy_GBps": 2 * n * 8 / tc / 1e9,
+                      "sum_read_GBps": n * 8 / ts / 1e9}))
+
+
 def precision_report(checkpoints, scale=1):
     """The mixed-precision run against the all-FP64 run of the same workload,
     both on the GPU from the same initial state: relative L2 and Linf
@@ -280,6 +309,9 @@ def main():
     ap.add_argument("--ppe-report", type=str, default="", metavar="STEPS",
                     help="instead of the benchmark, run the FP64 simulation and solve the PPE "
                          "of each listed step (comma separated) with every solver variant")
+    ap.add_argument("--stream-bandwidth", type=int, default=0, metavar="MIB",
+                    help="instead of the benchmark, time a read-only sum and a device copy of "
+                         "MIB MiB (what plain streams reach on this GPU)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="instead of the benchmark, run 5 plain device copies of MIB MiB "
                          "(reference reading for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -289,6 +321,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.stream_bandwidth:
+        stream_bandwidth(args.stream_bandwidth)
         return
     if args.precision_report:
         precision_report([int(s) for s in args.precision_report.split(",")], args.scale)
