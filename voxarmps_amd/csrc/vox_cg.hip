@@ -734,7 +734,7 @@ static int solve_plain(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 // Learned solve: flexible CG with the network's preconditioner, held to twice
 // the iterations of the last Jacobi-CG solve (the baseline); a first solve
 // without a baseline, a solve over the cap and a breakdown run Jacobi-CG,
-// which is also the judge of every result (same tolerance on the same system).
+// which sets the bar every result must meet (same tolerance on the same system).
 static int solve_learned(vox_ctx* c, vox_solve_info* info)
 {
     if (c->lpc_baseline <= 0) {
