@@ -128,7 +128,14 @@ typedef struct vox_params {
     int32_t matopt;       /* 1: higher-order Laplacian in MATRIX (functions.cu:901-914) */
     int32_t srcopt;       /* 1: higher-order source with error compensation
                              (functions.cu:928-988)                              */
-    int32_t reserved2;
+    int32_t ref_gpu;      /* 1: the reference's CUDA path instead of its CPU path
+                             (2D, FP64): neighbour cells and radius re+DELTA with
+                             the particle itself in its list
+                             (neighbour_parallel.cu:37-72, 145-199); MATRIX on
+                             x* and n* (functions.cu:1007-1133, main.cu:1165);
+                             gradient with 2 p^_i (382-385); BOUNDARIE with the
+                             collision factor cc and Ymax - DL/2 (2023-2047);
+                             PREPDATA copies u of every particle (2748-2782)   */
     double  wc_beta;      /* method 3: free-surface PND ratio of the equation of
                              state (BETA, main.cu:262)                           */
 } vox_params;

@@ -99,6 +99,7 @@ int orc_create(orc_ctx** out, const vox_params* p, int tp, int gp, int wp, int f
     if (!out || !p || tp <= 0 || gp < 0 || wp < 0 || fp < 0 || gp + wp + fp != tp)
         return VOX_ERR_ARG;
     if (p->dim != 2 && p->dim != 3) return VOX_ERR_ARG;
+    if (p->ref_gpu && p->dim != 2) return VOX_ERR_ARG;
     orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
     if (!c) return VOX_ERR_ARG;
     c->p = *p;
@@ -172,8 +173,9 @@ static grid_t make_grid(const vox_params* p, double re)
     grid_t g;
     g.hx = re;                                                  /* 2790-2791 */
     g.hz = re + p->delta;                                       /* 2796 */
-    g.ncx = (int)((p->xmax - p->xmin) / re) + 1;
-    g.ncy = (int)((p->ymax - p->ymin) / re) + 1;
+    if (p->ref_gpu) g.hx = re + p->delta;    /* CUDA path: neighbour_parallel.cu:46-47 */
+    g.ncx = (int)((p->xmax - p->xmin) / g.hx) + 1;
+    g.ncy = (int)((p->ymax - p->ymin) / g.hx) + 1;
     g.ncz = (p->dim == 3) ? (int)((p->zmax - p->zmin) / g.hz) + 1 : 1;
     return g;
 }
@@ -197,11 +199,12 @@ static int cell_coords(const vox_params* p, const grid_t* g, double x, double y,
     return 1;
 }
 
-/* accept rule: 3D d <= re+DELTA with self (2886); 2D I!=J and d < re (2907-2910) */
+/* accept rule: 3D d <= re+DELTA with self (2886); 2D I!=J and d < re (2907-2910);
+ * the CUDA path's 2D search: R <= re+DELTA with self (neighbour_parallel.cu:180-181) */
 static inline int accept(const orc_ctx* c, double re, const double* x, const double* y,
                          const double* z, int i, int j)
 {
-    if (c->p.dim == 3) return dist(c, x, y, z, i, j) <= re + c->p.delta;
+    if (c->p.dim == 3 || c->p.ref_gpu) return dist(c, x, y, z, i, j) <= re + c->p.delta;
     if (i == j) return 0;
     return dist(c, x, y, z, i, j) < re;
 }
@@ -629,8 +632,12 @@ int orc_ppe_assemble(orc_ctx* c)
 {
     const vox_params* p = &c->p;
     const orc_list* L = LL(c);
-    const double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
-    const double* n = c->f[VOX_F_N];
+    /* CPU path at x with n (main.cu:1149-1150); CUDA path at x* with n*
+     * (main.cu:1165, matrixCalc/sourceCalc functions.cu:1007-1133) */
+    const int star = p->ref_gpu != 0;
+    const double *x = c->f[star ? VOX_F_XSTAR : VOX_F_X], *y = c->f[star ? VOX_F_YSTAR : VOX_F_Y];
+    const double *z = c->f[star ? VOX_F_ZSTAR : VOX_F_Z];
+    const double* n = c->f[star ? VOX_F_NSTAR : VOX_F_N];
     double* src = c->f[VOX_F_SOURCE];
     int tp = c->tp;
     free(c->aval);
@@ -1079,7 +1086,10 @@ int orc_pressgrad(orc_ctx* c)
             double D = dist(c, x, y, z, i, j);
             if (i == j) continue;
             double wv = wfun(D, p->ktype, p->re_s);
-            double dp = (p->kh_correction == 1) ? (pn[j] + pn[i] - (ph[i] + ph[j])) : (pn[j] - ph[i]);
+            double dp;
+            if (p->kh_correction == 1 && p->ref_gpu) dp = pn[j] + pn[i] - 2.0 * ph[i];   /* 382-385 */
+            else if (p->kh_correction == 1) dp = pn[j] + pn[i] - (ph[i] + ph[j]);
+            else dp = pn[j] - ph[i];
             s1 = s1 + dp * (x[j] - x[i]) * wv / D / D;
             s2 = s2 + dp * (y[j] - y[i]) * wv / D / D;
             if (dim == 3) s3 = s3 + dp * (z[j] - z[i]) * wv / D / D;
@@ -1107,12 +1117,20 @@ int orc_euler(orc_ctx* c)
     const vox_params* p = &c->p;
     double *x = c->f[VOX_F_X], *y = c->f[VOX_F_Y], *z = c->f[VOX_F_Z];
     double *u = c->f[VOX_F_UNEW], *v = c->f[VOX_F_VNEW], *w = c->f[VOX_F_WNEW];
-    double DL = p->dl, CC = p->vel_cor;
+    /* the CUDA path passes the collision factor CC instead of velCor (main.cu:1293) */
+    double DL = p->dl, CC = p->ref_gpu ? p->cc : p->vel_cor;
     #pragma omp parallel for schedule(static)
     for (int i = c->gp + c->wp; i < c->tp; ++i) {
         x[i] = x[i] + u[i] * p->dt;
         y[i] = y[i] + v[i] * p->dt;
         if (p->dim == 3) z[i] = z[i] + w[i] * p->dt;
+        if (p->ref_gpu) {                                   /* eulerIntegration 2023-2047 */
+            if (y[i] < p->ymin + DL * 3.0) { v[i] = fabs(v[i]) * CC; y[i] = p->ymin + DL * 3.0; }
+            if (y[i] > p->ymax - DL / 2.0) { v[i] = -fabs(v[i]) * CC; y[i] = p->ymax - DL / 2.0; }
+            if (x[i] < p->xmin + DL * 3.0) { u[i] = fabs(u[i]) * CC; x[i] = p->xmin + DL * 3.0; }
+            if (x[i] > p->xmax - DL * 3.0) { u[i] = -fabs(u[i]) * CC; x[i] = p->xmax - DL * 3.0; }
+            continue;                                       /* 2D only */
+        }
         if (p->test_kind == VOX_TEST_DROP) continue;
         if (p->test_kind != VOX_TEST_TURB) {
             if (y[i] < p->ymin + DL * 3.0) { v[i] = fabs(v[i]) * CC; y[i] = p->ymin + DL * 3.0; }
@@ -1193,6 +1211,7 @@ int orc_prepdata(orc_ctx* c)
     double *u = c->f[VOX_F_U], *v = c->f[VOX_F_V], *w = c->f[VOX_F_W], *pp = c->f[VOX_F_P];
     int tp = c->tp, dim = p->dim, nanc = 0, oob = 0;
     int first = (p->test_kind == VOX_TEST_DROP) ? tp - c->fp - 1 : tp - c->fp;
+    if (p->ref_gpu) first = 0;                 /* prepData copies u of every particle (2753) */
     #pragma omp parallel for schedule(static) reduction(+ : nanc, oob)
     for (int i = 0; i < tp; ++i) {
         pp[i] = c->f[VOX_F_PNEW][i];

@@ -103,6 +103,7 @@ def test_create_rejects_bad_arguments():
     ("lpc_cap_pct", 0, _abi.VOX_ERR_ARG, b"lpc_cap_pct"),
     ("list_mode", 3, _abi.VOX_ERR_ARG, b"list_mode"),
     ("ir_maxouter", 0, _abi.VOX_ERR_ARG, b"ir_maxouter"),
+    ("ref_gpu", 2, _abi.VOX_ERR_ARG, b"ref_gpu"),
 ])
 def test_create_validates_options(field, value, code, text):
     """Options outside their documented values are refused before any device
@@ -114,6 +115,19 @@ def test_create_validates_options(field, value, code, text):
     ctx = C.c_void_p()
     rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 3, 1, 1, 1)
     assert rc == code and text in be.lib.vox_last_error()
+
+
+@pytest.mark.parametrize("dim,precision", [(3, 0), (2, 1)])
+def test_ref_gpu_is_2d_fp64_only(dim, precision):
+    """The reference's CUDA path exists in 2D only; its mode runs in FP64."""
+    be = vx.hip_backend()
+    p = vx.default_params(dim, 0.01)
+    p.xmin, p.xmax, p.ymin, p.ymax, p.zmin, p.zmax = 0.0, 1.0, 0.0, 1.0, 0.0, 1.0
+    p.ref_gpu = 1
+    p.precision = precision
+    ctx = C.c_void_p()
+    rc = be.lib.vox_create(C.byref(ctx), C.byref(p), 3, 1, 1, 1)
+    assert rc == _abi.VOX_ERR_ARG and b"ref_gpu" in be.lib.vox_last_error()
 
 
 def test_lpc_entry_points_check_arguments():

@@ -71,6 +71,9 @@ def _cases():
     r3 = vx.dam_break(3, 0.01, (10, 12, 8), (24, 16, 9))
     p3 = vx.set_box(vx.default_params(3, 0.01), r3)      # reference single radius 3.2 DL
     yield "3d-ref-radius", r3, p3
+    pg = vx.dam2d_params(j0)
+    pg.ref_gpu = 1                                       # the reference's CUDA search
+    yield "2d-ref-gpu", j0, pg
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["auto", "per-particle", "per-cell"])
@@ -120,6 +123,9 @@ def _stage_cases():
     yield "2d", j0, vx.dam2d_params(j0, tol=1e-10)
     c3 = vx.jitter(vx.dam_break(3, 0.005, (16, 16, 12), (48, 20, 13)), 0.1 * 0.005, seed=2)
     yield "3d", c3, vx.dam3d_params(c3, tol=1e-10)
+    pg = vx.dam2d_params(j0, tol=1e-10)
+    pg.ref_gpu = 1                                       # the reference's CUDA path
+    yield "2d-ref-gpu", j0, pg
 
 
 @pytest.mark.parametrize("name,case,p", list(_stage_cases()), ids=lambda v: v if isinstance(v, str) else "")
@@ -327,6 +333,28 @@ def test_trajectory_2d(hip, oracle, nsteps):
         scale = max(np.max(np.abs(b)), 1.0)
         assert np.max(np.abs(a - b)) <= 1e-6 * scale, n
     _assert_lists(g, c, "trajectory end")
+    g.close()
+    c.close()
+
+
+def test_trajectory_2d_ref_gpu(hip, oracle):
+    """The reference's CUDA path (ref_gpu): 40 steps of the jittered 2D dam
+    break against the oracle's restatement of that path."""
+    case = vx.jitter(vx.dam_break_2d_c0(), 0.05 * 0.01, seed=4)
+    p = vx.dam2d_params(case, tol=1e-10)
+    p.ref_gpu = 1
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    g.step(40)
+    c.step(40)
+    for n in ("X", "Y", "U", "V", "P"):
+        a, b = g.get(n), c.get(n)
+        scale = max(np.max(np.abs(b)), 1.0)
+        assert np.max(np.abs(a - b)) <= 1e-6 * scale, n
+    _assert_lists(g, c, "trajectory end")
+    cnt, ids = g.neighbors(vx.LIST_S)
+    assert all(i in ids[i, :cnt[i]] for i in range(0, case.tp, 97))   # the particle itself
     g.close()
     c.close()
 

@@ -568,3 +568,167 @@ def test_reference_iccg_solver(oracle, dim):
     assert np.max(np.abs(pi - xd)) <= 1e-12 * sc
     assert np.max(np.abs(pj - pi)) <= 1e-11 * sc
     assert info.iterations <= jac.iterations
+
+
+# ---- the reference's CUDA path (ref_gpu = 1): neighbour_parallel.cu, matrixCalc /
+# sourceCalc, pressGrad, eulerIntegration and prepData of functions.cu
+
+def test_ref_gpu_neighbor_known_answer(oracle):
+    """CUDA search (neighbour_parallel.cu:37-72, 145-199): cells re+DELTA wide
+    (ncx = int(4/1.2)+1 = 4), R <= re+DELTA with the particle itself, x-offset
+    outer, y-offset inner, ascending id in a cell. The CPU rule (d < re, self
+    excluded, cells re wide) on the same points for contrast."""
+    l0 = 1.0
+    x = np.array([0.5, 1.6, 2.35, 0.55])
+    y = np.array([0.5, 0.5, 0.5, 1.45])
+    pt = np.ones(4, np.int32)
+    case = vx.Case(2, l0, x, y, np.zeros(4), *(np.zeros(4) for _ in range(4)), pt)
+    p = vx.default_params(2, l0)
+    p.re_s = p.re_l = 1.0
+    p.delta = 0.2
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 4.0, 0.0, 4.0
+    p.ref_gpu = 1
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.NEIGHBOR()
+        cnt, ids = s.neighbors(vx.LIST_S)
+        cells = s.cells(vx.LIST_S)
+    # x cells int(x/1.2)+1 = 1, 2, 2, 1; y cells 1, 1, 1, 2 (key = ic-1 + (jc-1) 4)
+    assert list(cells) == [0, 1, 1, 4]
+    # 0: own cell, then (x+0, y+1): 3 at 0.951, then (x+1, y+0): 1 at 1.1 (2 at 1.85 is not)
+    assert list(ids[0, :cnt[0]]) == [0, 3, 1]
+    # 1: (x-1, y+0): 0; (x-1, y+1): 3 at 1.416 is not; own cell: 1, 2
+    assert list(ids[1, :cnt[1]]) == [0, 1, 2]
+    assert list(ids[2, :cnt[2]]) == [1, 2]
+    # 3: (x+0, y-1): 0, then its own cell
+    assert list(ids[3, :cnt[3]]) == [0, 3]
+    p.ref_gpu = 0
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.NEIGHBOR()
+        cnt, ids = s.neighbors(vx.LIST_S)
+    assert [list(ids[i, :cnt[i]]) for i in range(4)] == [[3], [2], [1], [0]]
+
+
+def test_ref_gpu_lattice_list_and_n0(oracle):
+    """Interior particle of a 2D lattice, re = 4 l0, DELTA = l0/5: the CUDA rule
+    lists the 44 offsets with |o| < 4, the 12 with 16 <= |o|^2 <= 17 and the
+    particle itself (57); W is 0 from re on, so n0 keeps its closed form."""
+    case = lattice(2, 0.01, 8)
+    p = params_for(case, 4.0)
+    p.delta = 0.2 * case.l0
+    p.ref_gpu = 1
+    c = centre_index(case)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        cnt, ids = s.neighbors(vx.LIST_S)
+        n0s, _ = s.n0()
+    row = ids[c, :cnt[c]]
+    assert cnt[c] == 57 and c in row
+    d2 = np.round(((case.x[row] - case.x[c]) ** 2 + (case.y[row] - case.y[c]) ** 2) / 1e-4)
+    assert int(np.sum(d2 >= 16)) == 12 and d2.max() == 17
+    assert abs(n0s - N0_KNOWN[0][2]) <= 1e-11 * N0_KNOWN[0][2]
+
+
+def test_ref_gpu_matrix_at_predicted_state(oracle):
+    """matrixCalc / sourceCalc on x* and n* (main.cu:1165): the CUDA-path system
+    equals the CPU-path system assembled after x := x* and n := n*, and differs
+    from the one at the current x."""
+    case = vx.jitter(vx.dam_break_2d_c0(), 0.05 * 0.01, seed=5)
+    p = vx.dam2d_params(case)
+    p.ref_gpu = 1
+    rng = np.random.default_rng(3)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        for st in ("PNUM", "NEIGHBOR", "V_FRACTION", "VISCOSITY"):
+            getattr(s, st)()
+        s.set("U", rng.normal(0, 0.2, case.tp))
+        s.set("V", rng.normal(0, 0.2, case.tp))
+        s.PREDICTION()
+        s.PNUMSTAR()
+        s.BCON()
+        s.MATRIX()
+        A1, src1 = s.ppe(), s.get("SOURCE")
+        q = p.copy()
+        q.ref_gpu = 0
+        s.set_params(q)
+        s.MATRIX()
+        A0 = s.ppe()
+        for a, b in (("X", "XSTAR"), ("Y", "YSTAR"), ("N", "NSTAR")):
+            s.set(a, s.get(b))
+        s.MATRIX()
+        A2, src2 = s.ppe(), s.get("SOURCE")
+    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+        assert np.array_equal(A1[k], A2[k]), k
+    assert np.array_equal(src1, src2)
+    assert not np.array_equal(A1["vals"], A0["vals"])
+
+
+def test_ref_gpu_pressgrad_two_phat_i(oracle):
+    """KH = 1 with p uniform and p^ = c + b x at the lattice centre (x = 0):
+    the CPU form p_j + p_i - (p^_i + p^_j) leaves -b sum dx^2 W / r^2 = -b n0/2,
+    u_new = u* + relaxp dt b / rho; the CUDA form p_j + p_i - 2 p^_i is constant
+    over the row, so the symmetric lattice sum vanishes."""
+    case, p = _fluid_lattice(2)
+    b, cst = 3.0e4, 17.0
+    out = {}
+    for mode in (0, 1):
+        p.ref_gpu = mode
+        with vx.Simulation(p, case, backend=oracle) as s:
+            s.init()
+            for t, f in (("XSTAR", "X"), ("YSTAR", "Y"), ("ZSTAR", "Z")):
+                s.set(t, s.get(f))
+            s.set("PNEW", np.full(case.tp, 800.0))
+            s.set("PHAT", cst + b * case.x)
+            s.set("USTAR", np.zeros(case.tp))
+            s.PRESSGRAD()
+            out[mode] = s.get("UNEW")[centre_index(case)]
+    want = p.relaxp * p.dt * b / p.rho1
+    assert abs(out[0] - want) <= 1e-9 * abs(want)
+    assert abs(out[1]) <= 1e-9 * abs(want)
+
+
+def test_ref_gpu_euler_and_prepdata(oracle):
+    """eulerIntegration (functions.cu:2023-2047): the collision factor CC and the
+    upper bound Ymax - DL/2 for every test kind; prepData copies u of every
+    particle (2753)."""
+    l0 = 0.01
+    case = vx.Case(2, l0, np.array([0.0, 0.05, 0.1]), np.array([0.0, 0.05, 0.05]), np.zeros(3),
+                   np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3),
+                   np.array([0, 1, 1], np.int32))
+    p = vx.default_params(2, l0)
+    p.xmin, p.xmax, p.ymin, p.ymax = -0.01, 0.3, -0.01, 0.1
+    p.dt = 1e-3
+    p.ref_gpu = 1
+    p.test_kind = vx._abi.TEST_DROP            # ignored by the CUDA kernel
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        # Ymax - DL/2 = 0.095: particle 1 ends inside at y = 0.09, particle 2 would
+        # end at 0.098 and is clamped
+        s.set("UNEW", np.array([5.0, 0.0, 0.0]))
+        s.set("VNEW", np.array([0.0, 40.0, 48.0]))
+        s.EULERINTEGRATION()
+        y, v = s.get("Y"), s.get("VNEW")
+        assert y[1] == 0.05 + 40.0 * 1e-3 and v[1] == 40.0        # CPU path: clamped at 0.07
+        assert y[2] == p.ymax - l0 / 2.0 and v[2] == -48.0 * p.cc
+        s.set("U", np.zeros(3))
+        s.PREPDATA()
+        assert np.array_equal(s.get("U"), s.get("UNEW"))          # the wall's u too
+    p.ref_gpu = 0
+    p.test_kind = vx._abi.TEST_DAM
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("UNEW", np.array([5.0, 0.0, 0.0]))
+        s.set("VNEW", np.array([0.0, 40.0, 48.0]))
+        s.EULERINTEGRATION()
+        y, v = s.get("Y"), s.get("VNEW")
+        assert y[1] == p.ymax - l0 * 3.0 and v[1] == -40.0 * p.vel_cor
+        s.set("U", np.zeros(3))
+        s.PREPDATA()
+        assert s.get("U")[0] == 0.0                                 # walls keep u
+
+
+def test_ref_gpu_is_2d_only(oracle):
+    case = vx.dam_break_3d_c1(16)
+    p = vx.dam3d_params(case)
+    p.ref_gpu = 1
+    with pytest.raises(RuntimeError):
+        vx.Simulation(p, case, backend=oracle)

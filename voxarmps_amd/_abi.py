@@ -57,7 +57,7 @@ class VoxParams(C.Structure):
         ("cg_maxit", _i32), ("cg_check", _i32), ("cap_s", _i32), ("cap_l", _i32),
         ("ir_eta", _f64), ("ir_maxouter", _i32), ("warm_start", _i32),
         ("list_mode", _i32), ("lpc_cap_pct", _i32),
-        ("method", _i32), ("matopt", _i32), ("srcopt", _i32), ("reserved2", _i32), ("wc_beta", _f64),
+        ("method", _i32), ("matopt", _i32), ("srcopt", _i32), ("ref_gpu", _i32), ("wc_beta", _f64),
     ]
 
     def copy(self):

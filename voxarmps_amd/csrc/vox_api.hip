@@ -89,24 +89,25 @@ int read_flags(vox_ctx* c)
     return VOX_OK;
 }
 
-// Lattice count of neighbours inside the accept radius, used to size rows.
-static int lattice_count(int dim, double dl, double re, double delta)
+// Lattice count of neighbours inside the accept radius, used to size rows
+// (incl: the inclusive rule of 3D and of the CUDA path, self counted).
+static int lattice_count(int dim, int incl, double dl, double re, double delta)
 {
-    double R = (dim == 3) ? re + delta : re;
+    double R = incl ? re + delta : re;
     int m = (int)std::ceil(R / dl) + 1;
     int cnt = 0;
     for (int a = -m; a <= m; ++a)
         for (int b = -m; b <= m; ++b)
             for (int cc = (dim == 3 ? -m : 0); cc <= (dim == 3 ? m : 0); ++cc) {
                 double d = std::sqrt((double)(a * a + b * b + cc * cc)) * dl;
-                if (dim == 3 ? d <= R : (d < R && (a || b))) ++cnt;
+                if (incl ? d <= R : (d < R && (a || b))) ++cnt;
             }
     return cnt;
 }
 
-static int auto_cap(int dim, double dl, double re, double delta)
+static int auto_cap(int dim, int incl, double dl, double re, double delta)
 {
-    int n = lattice_count(dim, dl, re, delta);
+    int n = lattice_count(dim, incl, dl, re, delta);
     int cap = n + n / 3 + 8;
     return (cap + 7) & ~7;
 }
@@ -213,7 +214,7 @@ int vox_params_default(vox_params* p, int dim, double dl)
     p->method = 1;
     p->matopt = 0;
     p->srcopt = 0;
-    p->reserved2 = 0;
+    p->ref_gpu = 0;
     p->wc_beta = 0.92;                              // main.cu:262
     return VOX_OK;
 }
@@ -262,6 +263,9 @@ static int validate(const vox_params* p)
         return fail(VOX_ERR_NOLIB, "PRESSURECALC method must be 1 (PPE) or 3 (equation of state)");
     if ((p->matopt != 0 && p->matopt != 1) || (p->srcopt != 0 && p->srcopt != 1))
         return fail(VOX_ERR_ARG, "matopt and srcopt must be 0 or 1");
+    if (p->ref_gpu != 0 && p->ref_gpu != 1) return fail(VOX_ERR_ARG, "ref_gpu must be 0 or 1");
+    if (p->ref_gpu && (p->dim != 2 || p->precision != VOX_PREC_FP64))
+        return fail(VOX_ERR_ARG, "ref_gpu (the reference's CUDA path) is 2D and FP64 only");
     if (p->ktype != 2 && p->ktype != 6 && p->ktype != 1 && p->ktype != 5)
         return fail(VOX_ERR_ARG, "unsupported kernel type");
     return VOX_OK;
@@ -301,7 +305,8 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
             return bail(rc);
         double re = (w == 0) ? c->p.re_s : c->p.re_l;
         int want = (w == 0 ? c->p.cap_s : c->p.cap_l);
-        if (want <= 0) want = auto_cap(c->p.dim, c->p.dl, re, c->p.delta);
+        if (want <= 0)
+            want = auto_cap(c->p.dim, c->p.dim == 3 || c->p.ref_gpu, c->p.dl, re, c->p.delta);
         if ((rc = ensure_list_storage(c, w, want))) return bail(rc);
     }
     for (int k = 0; k < 6; ++k)

@@ -85,6 +85,7 @@ struct Grid {
     double xmin, ymin, zmin;
     double hx, hz;          // cell width in x/y and in z (re and re+DELTA)
     int ncx, ncy, ncz, dim;
+    int incl;               // accept d <= R with the particle itself (3D; the CUDA path)
     long tnc;
 };
 

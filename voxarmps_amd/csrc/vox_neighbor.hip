@@ -10,6 +10,10 @@
 //           binning of 2820-2852 is a stable sort by cell),
 //   rule  = 3D: sqrt(dx^2+dy^2+dz^2) <= re+DELTA, self included (2886);
 //           2D: i != j and sqrt(dx^2+dy^2) < re (2907-2910).
+// With ref_gpu the 2D list follows the reference's CUDA search instead
+// (neighbour_parallel.cu:37-72, 145-199): cells re+DELTA wide in x and y,
+// sqrt(dx^2+dy^2) <= re+DELTA with the particle itself, the same stencil order,
+// ascending id inside a cell (its sort by cell key keeps the id order).
 // The distance is evaluated exactly as the reference (no FMA: the library is
 // built with -ffp-contract=off); a +-1e-10 relative band around the radius
 // decides which candidates need the correctly rounded sqrt.
@@ -155,8 +159,11 @@ static Grid make_grid(const vox_params& p, double re)
     g.hx = re;
     g.hz = re + p.delta;
     g.dim = p.dim;
-    g.ncx = (int)((p.xmax - p.xmin) / re) + 1;                    // functions.cu:2790-2791
-    g.ncy = (int)((p.ymax - p.ymin) / re) + 1;
+    g.incl = (p.dim == 3 || p.ref_gpu) ? 1 : 0;
+    // the CUDA path's 2D cells are re+DELTA wide (neighbour_parallel.cu:46-47, 57-58)
+    if (p.ref_gpu) g.hx = re + p.delta;
+    g.ncx = (int)((p.xmax - p.xmin) / g.hx) + 1;                  // functions.cu:2790-2791
+    g.ncy = (int)((p.ymax - p.ymin) / g.hx) + 1;
     g.ncz = (p.dim == 3) ? (int)((p.zmax - p.zmin) / g.hz) + 1 : 1; // 2796
     g.tnc = (long)g.ncx * g.ncy * g.ncz;
     return g;
@@ -251,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
     }
 }
 
-// Per-particle stencil walk. R is the accept radius (3D: re+DELTA, 2D: re).
+// Per-particle stencil walk. R is the accept radius (3D and the CUDA path:
+// re+DELTA, inclusive; 2D: re, exclusive).
 __device__ __forceinline__ int wave_max_i32(int v)
 {
     for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
@@ -322,8 +330,8 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
                             const double d2 = dx * dx + dy * dy + dz * dz;
                             bool acc = d2 < lo;
                             if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
-                                acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
-                            acc = acc && (j0 + u < e) && (g.dim == 3 || cu.id != i);
+                                acc = g.incl ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                            acc = acc && (j0 + u < e) && (g.incl || cu.id != i);
                             if (acc) {
                                 ring[k & 15] = cu.id;
                                 ++k;
@@ -415,8 +423,8 @@ __global__ __launch_bounds__(kWave) void k_build_list_cells(
                                 const double d2 = dx * dx + dy * dy + dz * dz;
                                 bool acc = d2 < lo;
                                 if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
-                                    acc = (g.dim == 3) ? (sqrt(d2) <= R) : (sqrt(d2) < R);
-                                acc = acc && active && (u < nc) && (g.dim == 3 || cu.id != i);
+                                    acc = g.incl ? (sqrt(d2) <= R) : (sqrt(d2) < R);
+                                acc = acc && active && (u < nc) && (g.incl || cu.id != i);
                                 if (acc) {
                                     ring[k & 15] = cu.id;
                                     ++k;
@@ -502,7 +510,7 @@ static int build_one(vox_ctx* c, int which, double re)
                                                         c->f[VOX_F_Y], c->f[VOX_F_Z]);
         VOX_HIP(hipGetLastError());
     }
-    double R = (c->p.dim == 3) ? re + c->p.delta : re;
+    double R = L.g.incl ? re + c->p.delta : re;
     // Walk per cell when cells hold enough particles to fill most of a wave
     // (expected count = cell volume / DL^dim), else per particle.
     const double per_cell = (c->p.dim == 3)

@@ -334,7 +334,11 @@ int ppe_assemble(vox_ctx* c)
     a.n0s = c->n0s;
     a.L = view(c, VOX_LIST_L);
     a.acap = c->a_cap;
-    a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z]; a.pnd = c->f[VOX_F_N];
+    // the CPU path assembles at x with n (main.cu:1149-1150), the CUDA path at
+    // x* with n* (main.cu:1165, functions.cu:1007-1133)
+    const bool star = p.ref_gpu != 0;
+    a.x = c->f[star ? VOX_F_XSTAR : VOX_F_X]; a.y = c->f[star ? VOX_F_YSTAR : VOX_F_Y];
+    a.z = c->f[star ? VOX_F_ZSTAR : VOX_F_Z]; a.pnd = c->f[star ? VOX_F_NSTAR : VOX_F_N];
     a.bcon = c->bcon; a.uidx = c->uidx;
     a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
     a.tcol = c->tcol; a.tval = c->tval;
@@ -361,7 +365,7 @@ int ppe_assemble(vox_ctx* c)
         s.n = n; s.dim = p.dim; s.re = p.re_l; s.n0s = c->n0s; s.dt = p.dt;
         s.hs_coef = -1.25 / (c->n0l * p.dt);
         s.L = view(c, VOX_LIST_L);
-        s.x = c->f[VOX_F_X]; s.y = c->f[VOX_F_Y]; s.z = c->f[VOX_F_Z];
+        s.x = a.x; s.y = a.y; s.z = a.z;
         s.us = c->f[VOX_F_USTAR]; s.vs = c->f[VOX_F_VSTAR]; s.ws = c->f[VOX_F_WSTAR];
         s.nstar = c->f[VOX_F_NSTAR]; s.bcon = c->bcon; s.ptype = c->ptype;
         s.hs = c->scratch_d;

@@ -608,7 +608,11 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
             if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
             else D = sqrt(dx * dx + dy * dy);
             double wv = wfun_r(D, a.ktype, a.re, inv_re);
-            double dp = (a.kh == 1) ? (a.pn[j] + pi_ - (phi_ + a.ph[j])) : (a.pn[j] - phi_);
+            // kh 1: p_j + p_i - (p^_i + p^_j) (functions.cu:275-278); kh 2, the
+            // reference's CUDA kernel: p_j + p_i - 2 p^_i (382-385); kh 0: p_j - p^_i
+            double dp = (a.kh == 1)   ? (a.pn[j] + pi_ - (phi_ + a.ph[j]))
+                        : (a.kh == 2) ? (a.pn[j] + pi_ - 2.0 * phi_)
+                                      : (a.pn[j] - phi_);
             const double inv_D = 1.0 / D;                 // x / D / D via div_by
             s1 = s1 + div_by(div_by(dp * dx * wv, D, inv_D), D, inv_D);
             s2 = s2 + div_by(div_by(dp * dy * wv, D, inv_D), D, inv_D);
@@ -638,7 +642,8 @@ int st_pressgrad(vox_ctx* c)
     const vox_params& p = c->p;
     GradArgs a;
     a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype;
-    a.kh = p.kh_correction; a.re = p.re_s; a.gnum = p.coef2d ? 2.0 : (double)p.dim;
+    a.kh = (p.ref_gpu && p.kh_correction == 1) ? 2 : p.kh_correction;
+    a.re = p.re_s; a.gnum = p.coef2d ? 2.0 : (double)p.dim;
     a.dt = p.dt; a.n0 = c->n0s; a.relaxp = p.relaxp; a.vmax = p.vmax;
     a.L = view(c, VOX_LIST_S);
     a.x = c->f[VOX_F_XSTAR]; a.y = c->f[VOX_F_YSTAR]; a.z = c->f[VOX_F_ZSTAR];
@@ -671,6 +676,8 @@ __device__ __forceinline__ Box box_of(const int32_t* region, const double* rbox,
     return Box{b[0], b[1], b[2], b[3], b[4], b[5]};
 }
 
+constexpr int kEulerRefGpu = -1;     // EulerArgs::kind of the CUDA-path boundary rule
+
 __global__ void k_euler(EulerArgs a)
 {
     int i = a.first + blockIdx.x * blockDim.x + threadIdx.x;
@@ -686,7 +693,18 @@ __global__ void k_euler(EulerArgs a)
     double z = 0.0, w = 0.0;
     if (a.dim == 3) { w = a.w[i]; z = a.z[i] + w * a.dt; }
     const double DL = a.dl, CC = a.cc;
-    if (a.kind == VOX_TEST_DAM) {
+    if (a.kind == kEulerRefGpu) {
+        // the reference's CUDA kernel (functions.cu:2023-2047): every test
+        // kind, upper y bound Ymax - DL/2
+        if (y < a.ymin + DL * 3.0) { v = fabs(v) * CC; y = a.ymin + DL * 3.0; }
+        if (y > a.ymax - DL / 2.0) { v = -fabs(v) * CC; y = a.ymax - DL / 2.0; }
+        if (x < a.xmin + DL * 3.0) { u = fabs(u) * CC; x = a.xmin + DL * 3.0; }
+        if (x > a.xmax - DL * 3.0) { u = -fabs(u) * CC; x = a.xmax - DL * 3.0; }
+        if (a.dim == 3) {
+            if (z < a.zmin + DL * 3.0) { w = fabs(w) * CC; z = a.zmin + DL * 3.0; }
+            if (z > a.zmax - DL * 3.0) { w = -fabs(w) * CC; z = a.zmax - DL * 3.0; }
+        }
+    } else if (a.kind == VOX_TEST_DAM) {
         if (y < a.ymin + DL * 3.0) { v = fabs(v) * CC; y = a.ymin + DL * 3.0; }
         if (y > a.ymax - DL * 3.0) { v = -fabs(v) * CC; y = a.ymax - DL * 3.0; }
         if (x < a.xmin + DL * 3.0) { u = fabs(u) * CC; x = a.xmin + DL * 3.0; }
@@ -709,8 +727,11 @@ int st_euler(vox_ctx* c)
     ProfScope ps(c, KC_EULER);
     const vox_params& p = c->p;
     EulerArgs a;
-    a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim; a.kind = p.test_kind;
-    a.dt = p.dt; a.dl = p.dl; a.cc = p.vel_cor;
+    a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim;
+    // the CUDA path passes the collision factor CC where the CPU path passes
+    // velCor (main.cu:1293 and the GPU branch beside it)
+    a.kind = p.ref_gpu ? kEulerRefGpu : p.test_kind;
+    a.dt = p.dt; a.dl = p.dl; a.cc = p.ref_gpu ? p.cc : p.vel_cor;
     a.xmin = p.xmin; a.xmax = p.xmax; a.ymin = p.ymin; a.ymax = p.ymax; a.zmin = p.zmin; a.zmax = p.zmax;
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
     a.u = c->f[VOX_F_UNEW]; a.v = c->f[VOX_F_VNEW]; a.w = c->f[VOX_F_WNEW];
@@ -1105,7 +1126,7 @@ int st_collision(vox_ctx* c)
         a.dim = p.dim; a.coll = p.coll; a.cc = p.cc; a.rho1 = p.rho1; a.rho2 = p.rho2; a.dt = p.dt;
         // Once the screen radius covers the list's own accept radius every
         // listed pair is a candidate and the replay is exact for any motion.
-        const double list_r = (p.dim == 3) ? p.re_s + p.delta : p.re_s;
+        const double list_r = (p.dim == 3 || p.ref_gpu) ? p.re_s + p.delta : p.re_s;
         a.half_margin = (rc > list_r) ? HUGE_VAL : 0.45 * margin;
         a.total = c->coll_off + n; a.ci = c->coll_i; a.cj = c->coll_j; a.pt = c->ptype;
         a.cap = c->coll_cap;
@@ -1224,6 +1245,7 @@ int st_prepdata(vox_ctx* c)
     PrepArgs a;
     a.n = c->tp; a.dim = p.dim;
     a.first = (p.test_kind == VOX_TEST_DROP) ? c->tp - c->fp - 1 : c->tp - c->fp;
+    if (p.ref_gpu) a.first = 0;          // the CUDA kernel copies u of every particle (2753)
     a.xmin = p.xmin; a.xmax = p.xmax; a.ymin = p.ymin; a.ymax = p.ymax; a.zmin = p.zmin; a.zmax = p.zmax;
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
     a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW]; a.pn = c->f[VOX_F_PNEW];

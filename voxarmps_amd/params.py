@@ -55,6 +55,7 @@ def default_params(dim, dl):
     p.method = 1
     p.matopt = 0
     p.srcopt = 0
+    p.ref_gpu = 0
     p.wc_beta = 0.92
     return p
 
