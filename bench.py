@@ -52,6 +52,50 @@ def measured_traffic(precision, nu, nnz):
     return e["bytes_per_launch"]
 
 
+def list_counts(sim, which):
+    """Entries per particle of a neighbour list (counts only, no ids)."""
+    import numpy as np
+    cnt = np.empty(sim.tp, np.int32)
+    sim.be.check(sim.be.fn("get_neighbors")(sim.ctx, which, cnt.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            None, 0), "get_neighbors")
+    return cnt
+
+
+def step_bytes(sim, case, nu, nnz_total, iters, s=8):
+    """Compulsory HBM bytes of one time step, stage by stage: every array read
+    once and written once, neighbour-gathered fields counted once, the index
+    list of the radius used read once per consuming stage (4 bytes an entry
+    plus its count). Lists, unknowns, nonzeros and CG iterations are those of
+    the run. FP64 values (s = 8)."""
+    import numpy as np
+    import voxarmps_amd as vx
+    N, d = case.tp, case.dim
+    fl = case.ptype > 0
+    F = int(np.count_nonzero(fl))
+    cs = list_counts(sim, vx.LIST_S).astype(np.int64) + 1
+    cl = list_counts(sim, vx.LIST_L).astype(np.int64) + 1
+    bc = sim.get("BCON")
+    S, Lsum = int(cs.sum()), int(cl.sum())
+    SF, LF, LU = int(cs[fl].sum()), int(cl[fl].sum()), int(cl[bc == 0].sum())
+    off = nnz_total - nu
+    parts = {
+        "neighbour lists": 2 * N * d * 8 + 4 * (S + Lsum),
+        "pnd (n, n*)": 2 * (4 * S + N * (d * s + s)),
+        "volume fraction": 4 * S + N * (d * s + 4) + N * s,
+        "viscosity": N * (4 + s),
+        "predictor": 4 * LF + N * (2 * d * s + 2 * s + 4) + F * (3 * s + 2 * d * s),
+        "bcon": N * (8 + s),
+        "assembly": 4 * LU + N * (d * s + 4) + nu * 2 * s + off * (4 + s) + 4 * (nu + 1),
+        "cg": iters * (off * (4 + s) + 4 * (nu + 1) + 13 * s * nu),
+        "phat": 4 * S + 2 * N * s,
+        "gradient": 4 * SF + N * (d * s + 2 * s) + F * (2 * d * s + 2 * s),
+        "euler": 4 * F * d * s,
+        "collision": 4 * S + 2 * N * d * s + 2 * F * d * s,
+        "prepdata": N * (2 * s + 2 * d * s),
+    }
+    return sum(parts.values()), parts
+
+
 def spmv_bytes(nu, nnz_total, vbytes=8):
     """Compulsory bytes of one q = A p launch: off-diagonal values (vbytes)
     and column ids (4 B), diagonal (8 B, FP64 in both modes), row length
@@ -391,6 +435,9 @@ def main():
     info = sim.ppe()
     nu = int(info["rowptr"].shape[0] - 1)
     nnz = int(info["rowptr"][-1])
+    del info
+    step_total, step_parts = (None, None) if (ensemble or mixed) else \
+        step_bytes(sim, case, nu, nnz, sum(iters) / len(iters))
     ms = C.c_double(0.0)
     nl = C.c_int64(0)
     be.lib.vox_prof_get(sim.ctx, b"spmv", C.byref(ms), C.byref(nl))
@@ -481,6 +528,14 @@ def main():
             "launches": nl.value,
             "unknowns": nu,
             "nnz": nnz,
+        },
+        "fluid_particle_steps_per_s": case.fp * args.steps / el,
+        "step_roofline": None if step_total is None else {
+            "bytes_per_step": step_total,
+            "achieved": step_total / (el / args.steps) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": step_total / (el / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "bytes_by_stage": step_parts,
         },
         "stage_ms_per_step": per_kernel,
         "ppe_solve": ppe,
