@@ -169,6 +169,17 @@ def test_grd_round_trip_through_c_reader(tmp_path):
         assert np.array_equal(back.x, c.x) and np.array_equal(back.ptype, c.ptype)
 
 
+def test_reference_input_fixtures_parse():
+    """The reference's 2D inputs, committed as fixtures: 6 columns per row,
+    particles ordered dummy, wall, fluid."""
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    c = vx.cases.read_grd(os.path.join(golden, "input-dam.grd"), 2)
+    assert c.tp == 1122 and (c.gp, c.wp, c.fp) == (162, 312, 648)
+    assert np.all(np.diff(c.ptype) >= 0)
+    d = vx.cases.read_grd(os.path.join(golden, "input-drop.grd"), 2)
+    assert d.tp == 1005 and (d.gp, d.wp, d.fp) == (0, 0, 1005)
+
+
 def test_reference_input_files_parse(tmp_path):
     """The reference's own .grd inputs hold 6 (2D) / 8 (3D) columns per row."""
     ref = "/root/reference/inputs"
