@@ -169,7 +169,7 @@ def test_grd_round_trip_through_c_reader(tmp_path):
         assert np.array_equal(back.x, c.x) and np.array_equal(back.ptype, c.ptype)
 
 
-def test_reference_input_fixtures_parse():
+def test_reference_input_fixtures_parse(tmp_path):
     """The reference's 2D inputs, committed as fixtures: 6 columns per row,
     particles ordered dummy, wall, fluid."""
     golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -178,6 +178,18 @@ def test_reference_input_fixtures_parse():
     assert np.all(np.diff(c.ptype) >= 0)
     d = vx.cases.read_grd(os.path.join(golden, "input-drop.grd"), 2)
     assert d.tp == 1005 and (d.gp, d.wp, d.fp) == (0, 0, 1005)
+    import gzip
+    import hashlib
+    for name, dim, counts in (("input-turb.grd", 2, (280, 560, 8970)),
+                              ("input-dam3d.grd", 3, (2732, 4384, 1024))):
+        with gzip.open(os.path.join(golden, name + ".gz"), "rb") as f:
+            data = f.read()
+        path = os.path.join(tmp_path, name)
+        with open(path, "wb") as f:
+            f.write(data)
+        e = vx.cases.read_grd(path, dim)
+        assert (e.gp, e.wp, e.fp) == counts and np.all(np.diff(e.ptype) >= 0)
+        assert hashlib.sha256(data).hexdigest() in open(os.path.join(golden, "README.md")).read()
 
 
 def test_reference_input_files_parse(tmp_path):
