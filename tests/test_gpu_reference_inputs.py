@@ -1,5 +1,5 @@
 """The reference's own inputs (tests/golden: its inputs/input-dam.grd,
-input-drop.grd, input-turb.grd and input-dam3d.grd) run with the reference's
+input-drop.grd, input-shear.grd, input-turb.grd and input-dam3d.grd) run with the reference's
 constants through the GPU library and through the oracle. Bar as for the other trajectories: positions,
 velocities and pressure within 1e-6 of their scale after 30 steps (100 for
 the periodic case; the solver tolerance is the only difference), neighbour
@@ -43,6 +43,15 @@ def _dam(tmp):
     return case, p
 
 
+def _shear(tmp):
+    """input-shear.grd, the shear-driven cavity: no gravity (main.cu:319-328);
+    the lid is the wall rows the file gives u = 0.5 and 1.0."""
+    case = read_golden("input-shear.grd", 2, tmp)
+    p = vx.set_box(vx.default_params(2, DL), case)
+    p.gx = p.gy = 0.0
+    return case, p
+
+
 def _turb(tmp):
     """input-turb.grd as main.cu runs the Poiseuille test: SPS on, no gravity,
     velCor 0.8 (305-322, 509-512), the periodic boundary of BOUNDARIE."""
@@ -78,8 +87,9 @@ def _drop(tmp):
 
 # the Poiseuille case runs 100 steps: its fastest fluid column (0.5 m/s, 2 DL
 # before Xmax) then crosses Xmax and wraps to Xmin
-@pytest.mark.parametrize("make,steps", [(_dam, 30), (_drop, 30), (_turb, 100), (_dam3d, 30)],
-                         ids=["input-dam", "input-drop", "input-turb", "input-dam3d"])
+@pytest.mark.parametrize("make,steps",
+                         [(_dam, 30), (_drop, 30), (_shear, 30), (_turb, 100), (_dam3d, 30)],
+                         ids=["input-dam", "input-drop", "input-shear", "input-turb", "input-dam3d"])
 def test_reference_input_trajectory(hip, oracle, make, steps, tmp_path):
     case, p = make(str(tmp_path))
     p.cg_tol = 1e-10
@@ -105,6 +115,9 @@ def test_reference_input_trajectory(hip, oracle, make, steps, tmp_path):
     assert (sg.nan_count, sg.out_of_box) == (sc.nan_count, sc.out_of_box) == (0, 0)
     if p.gy != 0.0:
         assert np.max(np.abs(c.get("P"))) > 0
+    if make is _shear:
+        f = case.ptype > 0
+        assert np.max(np.abs(c.get("U")[f])) > 0                # the lid drives the fluid
     if p.test_kind == vx._abi.TEST_TURB:
         assert np.count_nonzero(c.get("X") < x0 - 0.5) > 0      # periodic wrap exercised
     g.close()
