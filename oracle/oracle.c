@@ -576,25 +576,6 @@ int orc_prediction(orc_ctx* c)
             continue;
         }
         double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
-        /* Two radii: the pressure-gradient sums run over list S, the
-         * gradient's radius, in its order; with one radius both lists are
-         * the reference's single list and each sum keeps its order. */
-        const int split = c->two_lists;
-        if (split) {
-            const orc_list* S = LS(c);
-            const int32_t* rs = ROW(S, i);
-            for (int l = 0; l < S->cnt[i]; ++l) {
-                int j = rs[l];
-                double D = dist(c, x, y, z, i, j);
-                if (i != j && D > 0) {
-                    double wg = wfun(D, p->ktype, p->re_s);
-                    double dp = pn[j] - ph[i];
-                    s1 = s1 + dp * (x[j] - x[i]) * wg / D / D;
-                    s2 = s2 + dp * (y[j] - y[i]) * wg / D / D;
-                    if (dim == 3) s3 = s3 + dp * (z[j] - z[i]) * wg / D / D;
-                }
-            }
-        }
         const int32_t* r = ROW(L, i);
         for (int l = 0; l < L->cnt[i]; ++l) {
             int j = r[l];
@@ -610,11 +591,9 @@ int orc_prediction(orc_ctx* c)
                 double wg = wfun(D, p->ktype, p->re_s);
                 double wl = wfun(D, p->ktype, p->re_l);
                 double dp = pn[j] - ph[i];
-                if (!split) {
-                    s1 = s1 + dp * (x[j] - x[i]) * wg / D / D;
-                    s2 = s2 + dp * (y[j] - y[i]) * wg / D / D;
-                    if (dim == 3) s3 = s3 + dp * (z[j] - z[i]) * wg / D / D;
-                }
+                s1 = s1 + dp * (x[j] - x[i]) * wg / D / D;
+                s2 = s2 + dp * (y[j] - y[i]) * wg / D / D;
+                if (dim == 3) s3 = s3 + dp * (z[j] - z[i]) * wg / D / D;
                 s4 = s4 + wl * (u[j] - u[i]) * NEU;
                 s5 = s5 + wl * (v[j] - v[i]) * NEU;
                 if (dim == 3) s6 = s6 + wl * (w[j] - w[i]) * NEU;

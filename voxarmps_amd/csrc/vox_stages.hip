@@ -374,7 +374,6 @@ struct PredArgs {
     int n, dim, ktype;
     double re_s, re_l, rho1, rho2, gx, gy, gz, dt, lapc, ln0, gcoef_num, n0s, relax1;
     ListView L;
-    ListView LS;                 // the gradient's list (two radii)
     const int32_t* pt;
     const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
     double *xs, *ys, *zs, *us, *vs, *ws;
@@ -393,47 +392,6 @@ __global__ void k_pack_pred(PredArgs a)
     a.rec[i] = r;
 }
 
-// Pressure-gradient sums of the predictor over list S (two radii): the
-// gradient's own radius, in list-S order; the same operations as the
-// single-list loop below.
-__device__ __forceinline__ void pred_grad_sums(const PredArgs& a, int i, double xi, double yi,
-                                               double zi, double phi_, double inv_re_s,
-                                               double& s1, double& s2, double& s3)
-{
-    const int cnt = a.LS.cnt[i];
-    const int32_t* __restrict__ row = a.LS.ids + sl_index(i, 0, a.LS.cap);
-    for (int g = 0; (g << 3) < cnt; ++g) {
-        int32_t jc[8];
-        load_chunk8(row, g, jc);
-        constexpr int U = 8;
-        int jj[U];
-        double px[U], py[U], pz[U], pp[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + u < cnt) ? jc[u] : i;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const PredRec* r = a.rec + jj[u];
-            px[u] = r->x; py[u] = r->y; pz[u] = r->z; pp[u] = r->pn;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const double dx = px[u] - xi, dy = py[u] - yi, dz = pz[u] - zi;
-            const double D = sqrt(dx * dx + dy * dy + dz * dz);
-            const bool use = (jj[u] != i) && (D > 0);
-            const double inv_D = 1.0 / D;
-            const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
-            const double dp = pp[u] - phi_;
-            s1 = use ? s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D) : s1;
-            s2 = use ? s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D) : s2;
-            s3 = use ? s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D) : s3;
-        }
-    }
-}
-
-// SPLIT (two radii): the gradient sums run over list S (pred_grad_sums) and
-// the list-L loop keeps only the viscous sums. With one radius both lists are
-// the same list and every sum runs in the same order either way.
-template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
 {
     int i = xcd_tid();
@@ -452,7 +410,6 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
     const double inv_rhoN = 1.0 / rhoN;
     const double inv_re_s = 1.0 / a.re_s, inv_re_l = 1.0 / a.re_l;
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
-    if constexpr (SPLIT) pred_grad_sums(a, i, xi, yi, zi, phi_, inv_re_s, s1, s2, s3);
     const int cnt = a.L.cnt[i];
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
     // One row chunk (8 ids) per outer step, consumed in halves of 4: a half's
@@ -484,32 +441,22 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
         const double mj = rj.meu;
         // every quotient below equals the plain IEEE division (div_by); the
         // six divisions by D share one reciprocal
+        const double inv_D = 1.0 / D;
         const double hm = div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
         const double NEU = (ti == tj[u]) ? div_by(mi, rhoN, inv_rhoN) : hm;
+        const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
+        const double wl = (a.re_l == a.re_s) ? wg : wfun_r(D, a.ktype, a.re_l, inv_re_l);
+        const double dp = rj.pn - phi_;
         const double du = rj.u - ui, dv = rj.v - vi, dw = rj.w - wi;
-        if constexpr (SPLIT) {
-            const double wl = wfun_r(D, a.ktype, a.re_l, inv_re_l);
-            s4 = use ? s4 + wl * du * NEU : s4;
-            s5 = use ? s5 + wl * dv * NEU : s5;
-            s7 = use ? s7 + wl * du : s7;
-            s8 = use ? s8 + wl * dv : s8;
-            s6 = use ? s6 + wl * dw * NEU : s6;      // z sums: only read in 3D
-            s9 = use ? s9 + wl * dw : s9;
-        } else {
-            const double inv_D = 1.0 / D;
-            const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
-            const double wl = (a.re_l == a.re_s) ? wg : wfun_r(D, a.ktype, a.re_l, inv_re_l);
-            const double dp = rj.pn - phi_;
-            s1 = use ? s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D) : s1;
-            s2 = use ? s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D) : s2;
-            s4 = use ? s4 + wl * du * NEU : s4;
-            s5 = use ? s5 + wl * dv * NEU : s5;
-            s7 = use ? s7 + wl * du : s7;
-            s8 = use ? s8 + wl * dv : s8;
-            s3 = use ? s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D) : s3;   // z sums: only read in 3D
-            s6 = use ? s6 + wl * dw * NEU : s6;
-            s9 = use ? s9 + wl * dw : s9;
-        }
+        s1 = use ? s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D) : s1;
+        s2 = use ? s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D) : s2;
+        s4 = use ? s4 + wl * du * NEU : s4;
+        s5 = use ? s5 + wl * dv * NEU : s5;
+        s7 = use ? s7 + wl * du : s7;
+        s8 = use ? s8 + wl * dv : s8;
+        s3 = use ? s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D) : s3;   // z sums: only read in 3D
+        s6 = use ? s6 + wl * dw * NEU : s6;
+        s9 = use ? s9 + wl * dw : s9;
     }
     }
     }
@@ -542,7 +489,7 @@ int st_prediction(vox_ctx* c)
     a.gcoef_num = p.coef2d ? 2.0 : (double)p.dim;
     a.n0s = c->n0s;
     a.relax1 = (1 - p.relaxp);
-    a.L = view(c, VOX_LIST_L); a.LS = view(c, VOX_LIST_S); a.pt = c->ptype;
+    a.L = view(c, VOX_LIST_L); a.pt = c->ptype;
     a.x = c->f[VOX_F_X]; a.y = c->f[VOX_F_Y]; a.z = c->f[VOX_F_Z];
     a.u = c->f[VOX_F_U]; a.v = c->f[VOX_F_V]; a.w = c->f[VOX_F_W];
     a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.meu = c->f[VOX_F_MEU];
@@ -552,8 +499,7 @@ int st_prediction(vox_ctx* c)
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = c->pred_rec;
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    if (c->two_lists) k_prediction<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    else k_prediction<false><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    k_prediction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
