@@ -64,3 +64,29 @@ def test_no_unknowns(hip, precision, solver):
     s.close()
     if net is not None:
         net.close()
+
+
+def test_volume_fraction_single_and_two_phase(hip, oracle):
+    """With no PTYPE 2 particle the library writes C = +0 without the walk
+    (every fluid-2 sum is +0); the values equal the oracle's walk bit for bit.
+    Uploading types with a second fluid brings the walk back."""
+    case = vx.jitter(vx.dam_break_2d_c0(), 0.1 * 0.01, seed=8)
+    p = vx.dam2d_params(case)
+    g = vx.Simulation(p, case, backend=hip)
+    c = vx.Simulation(p, case, backend=oracle)
+    for s in (g, c):
+        s.init()
+        s.set("C", np.full(case.tp, 7.0))
+        s.V_FRACTION()
+    cg, cc = g.get("C"), c.get("C")
+    assert np.array_equal(cg, cc) and np.all(cg == 0) and not np.any(np.signbit(cg))
+    pt = case.ptype.copy()
+    f = pt > 0
+    pt[f & (case.y > np.median(case.y[f]))] = 2
+    for s in (g, c):
+        s.set("PTYPE", pt)
+        s.V_FRACTION()
+    cg, cc = g.get("C"), c.get("C")
+    assert np.array_equal(cg, cc) and np.count_nonzero(cg) > 0
+    g.close()
+    c.close()

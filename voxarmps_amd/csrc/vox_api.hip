@@ -404,6 +404,12 @@ int vox_upload(vox_ctx* c, int field, const void* host)
     size_t bytes = (int_field(field) ? sizeof(int32_t) : sizeof(double)) * (size_t)c->tp;
     VOX_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->stream));
     VOX_HIP(hipStreamSynchronize(c->stream));
+    if (field == VOX_F_PTYPE) {          // types never change during a run
+        const int32_t* t = static_cast<const int32_t*>(host);
+        long n2 = 0;
+        for (int i = 0; i < c->tp; ++i) n2 += (t[i] == 2);
+        c->n_fluid2 = n2;
+    }
     return VOX_OK;
 }
 

@@ -188,6 +188,7 @@ struct vox_ctx {
 
     double* f[VOX_F_COUNT] = {};
     int32_t* ptype = nullptr;
+    long n_fluid2 = -1;            // PTYPE 2 particles in the uploaded types (-1: not known)
     int32_t* bcon = nullptr;
 
     vox::DevList list[2];

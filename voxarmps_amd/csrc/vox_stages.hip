@@ -298,6 +298,9 @@ __global__ __launch_bounds__(kBlock) void k_vfraction(int n, int dim, int ktype,
 int st_vfraction(vox_ctx* c)
 {
     ProfScope ps(c, KC_VFRAC);
+    // Without fluid-2 particles every sum of fluid-2 weights is +0, so each C
+    // is 0 / s1 = +0 (or the s1 == 0 case, also +0): the same bits as the walk.
+    if (c->n_fluid2 == 0) return fill_f64(c, c->f[VOX_F_C], 0.0, c->tp);
     k_vfraction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
         c->tp, c->p.dim, c->p.ktype, c->p.re_s, view(c, VOX_LIST_S), c->ptype, c->f[VOX_F_X],
         c->f[VOX_F_Y], c->f[VOX_F_Z], c->f[VOX_F_C]);
