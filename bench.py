@@ -81,7 +81,8 @@ def step_bytes(sim, case, nu, nnz_total, iters, s=8):
     parts = {
         "neighbour lists": 2 * N * d * 8 + 4 * (S + Lsum),
         "pnd (n, n*)": 2 * (4 * S + N * (d * s + s)),
-        "volume fraction": 4 * S + N * (d * s + 4) + N * s,
+        # without a second fluid C is written directly (every fluid-2 sum is 0)
+        "volume fraction": (4 * S + N * (d * s + 4) + N * s) if np.any(case.ptype == 2) else N * s,
         "viscosity": N * (4 + s),
         "predictor": 4 * LF + N * (2 * d * s + 2 * s + 4) + F * (3 * s + 2 * d * s),
         "bcon": N * (8 + s),
