@@ -291,6 +291,13 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
     if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(VOX_ERR_HIP, "hipGetDevice"));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(VOX_ERR_HIP, "hipStreamCreate"));
+    if (c->two_lists) {
+        // the side stream of the neighbour build (default priority, like the main one)
+        if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(VOX_ERR_HIP, "side stream of the neighbour build"));
+    }
     const size_t n = (size_t)tp;
     for (int f = 0; f < VOX_F_COUNT; ++f) {
         if (f == VOX_F_PTYPE || f == VOX_F_BCON) continue;
@@ -312,7 +319,8 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
     for (int k = 0; k < 6; ++k)
         if ((rc = dalloc(&c->sps_s[k], n))) return bail(rc);
     if ((rc = dalloc(&c->scratch_d, n)) || (rc = dalloc(&c->pred_rec, n)) ||
-        (rc = dalloc(&c->pos_rec, n)) || (rc = dalloc(&c->blockmax, (size_t)grid_for(n))))
+        (rc = dalloc(&c->pos_rec, n)) || (rc = dalloc(&c->blockmax, (size_t)grid_for(n))) ||
+        (rc = dalloc(&c->blockmax2, (size_t)grid_for(n))))
         return bail(rc);
     if ((rc = dalloc(&c->uidx, n)) || (rc = dalloc(&c->urow, n)) || (rc = dalloc(&c->uflag, n)) ||
         (rc = dalloc(&c->rowlen, n)) || (rc = dalloc(&c->adiag, n)) || (rc = dalloc(&c->rhs, n)) ||
@@ -351,7 +359,7 @@ int vox_destroy(vox_ctx* c)
         dfree(L.order_raw); dfree(L.rec);
     }
     for (int k = 0; k < 6; ++k) dfree(c->sps_s[k]);
-    dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pos_rec); dfree(c->blockmax);
+    dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pos_rec); dfree(c->blockmax); dfree(c->blockmax2);
     dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->rowlen); dfree(c->acol);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs); dfree(c->tcol); dfree(c->tval);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
@@ -359,12 +367,15 @@ int vox_destroy(vox_ctx* c)
     dfree(c->coll_cnt); dfree(c->coll_off); dfree(c->coll_i); dfree(c->coll_j);
     dfree(c->coll_act); dfree(c->coll_cnt2); dfree(c->coll_off2); dfree(c->coll_i2); dfree(c->coll_j2);
     dfree(c->coll_stamp); dfree(c->coll_root); dfree(c->region); dfree(c->rbox); dfree(c->coll_logidx); dfree(c->coll_log); dfree(c->coll_logn);
-    dfree(c->scan_blocks); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
+    dfree(c->scan_blocks); dfree(c->scan_blocks2); dfree(c->partials); dfree(c->dscal); dfree(c->dflags);
     lpc_free_ctx(c);
     if (c->hflags) (void)hipHostFree(c->hflags);
     if (c->hscal) (void)hipHostFree(c->hscal);
     for (auto& r : c->prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->prof_free) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VOX_OK;

@@ -287,6 +287,13 @@ struct vox_ctx {
     int32_t* scan_blocks = nullptr;
     long scan_blocks_cap = 0;
     int32_t* blockmax = nullptr;   // one slot per 256-particle block
+    // side stream of the neighbour build (list L beside list S), with its own
+    // scan scratch and block-maximum slots, and the events joining the streams
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int32_t* scan_blocks2 = nullptr;
+    long scan_blocks2_cap = 0;
+    int32_t* blockmax2 = nullptr;
 
     // reductions
     double* partials = nullptr;    // 8 * kMaxPartials

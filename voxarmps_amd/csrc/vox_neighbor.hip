@@ -23,6 +23,8 @@
 // LDS tile) with position records copied into cell order -> per-particle stencil
 // walk (one thread per particle, or one wave per cell) writing sliced rows
 // (sl_index).
+#include <utility>
+
 #include "vox_internal.hpp"
 
 namespace vox {
@@ -472,7 +474,12 @@ int ensure_list_storage(vox_ctx* c, int which, int cap)
     return VOX_OK;
 }
 
-static int build_one(vox_ctx* c, int which, double re)
+static int search_one(vox_ctx* c, int which, double re);
+
+// Grid, cell sort and one search pass of list `which` on c->stream, without
+// waiting for it. Reports, once the stream has run, go to the list's overflow
+// and longest-row flags and to the shared domain flags.
+static int launch_one(vox_ctx* c, int which, double re)
 {
     DevList& L = c->list[which];
     const int n = c->tp;
@@ -488,8 +495,6 @@ static int build_one(vox_ctx* c, int which, double re)
         VOX_HIP(hipMalloc(&L.cell_fill, sizeof(int32_t) * (L.g.tnc + 1)));
         L.tnc_alloc = L.g.tnc;
     }
-    const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
-    const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
     {
         ProfScope ps(c, KC_CELL);
         VOX_TRY(fill_i32(c, L.cell_fill, 0, L.g.tnc));
@@ -510,6 +515,17 @@ static int build_one(vox_ctx* c, int which, double re)
                                                         c->f[VOX_F_Y], c->f[VOX_F_Z]);
         VOX_HIP(hipGetLastError());
     }
+    return search_one(c, which, re);
+}
+
+// One search pass over the sorted cells of list `which` (rows at the list's
+// current capacity), on c->stream.
+static int search_one(vox_ctx* c, int which, double re)
+{
+    DevList& L = c->list[which];
+    const int n = c->tp;
+    const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
+    const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
     double R = L.g.incl ? re + c->p.delta : re;
     // Walk per cell when cells hold enough particles to fill most of a wave
     // (expected count = cell volume / DL^dim), else per particle.
@@ -519,47 +535,67 @@ static int build_one(vox_ctx* c, int which, double re)
     const int mode = c->tuned_mode[which] ? c->tuned_mode[which] : c->p.list_mode;
     const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0)) &&
                            L.g.tnc * kCellHalves < (1L << 31);
+    ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
+    VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
+    if (cell_walk) {
+        VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
+        k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
+            L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+    } else {
+        const int nb = grid_for(n);
+        k_build_list<<<nb, kBlock, 0, c->stream>>>(
+            L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
+            L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
+        k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
+    }
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// After read_flags: the domain error, naming the first particle outside the
+// grid (its type, position and velocity).
+static int domain_error(vox_ctx* c)
+{
+    const int i = c->hflags[FLAG_DOMAIN_FIRST];
+    double v[5] = {0, 0, 0, 0, 0};
+    int32_t t = 0;
+    const int fs[5] = {VOX_F_X, VOX_F_Y, VOX_F_Z, VOX_F_U, VOX_F_V};
+    if (i >= 0 && i < c->tp) {
+        for (int k = 0; k < 5; ++k)
+            VOX_HIP(hipMemcpy(&v[k], c->f[fs[k]] + i, sizeof(double), hipMemcpyDeviceToHost));
+        VOX_HIP(hipMemcpy(&t, c->ptype + i, sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "; first: particle %d (PTYPE %d) at (%.6g, %.6g, %.6g), velocity (%.6g, %.6g)",
+                  i, t, v[0], v[1], v[2], v[3], v[4]);
+    return fail(VOX_ERR_DOMAIN, "particle outside the neighbour grid box (" +
+                                    std::to_string(c->hflags[FLAG_DOMAIN]) + " particles)" + buf);
+}
+
+// After read_flags: rows of list `which` that did not fit are searched again
+// with wider rows (up to 3 more passes).
+static int regrow_one(vox_ctx* c, int which, double re)
+{
+    const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
+    const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        {
-            ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
-            VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
-            if (cell_walk) {
-                VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-                k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
-                    L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
-            } else {
-                const int nb = grid_for(n);
-                k_build_list<<<nb, kBlock, 0, c->stream>>>(
-                    L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
-                    L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
-                k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
-            }
-            VOX_HIP(hipGetLastError());
-        }
-        VOX_TRY(read_flags(c));
-        if (c->hflags[FLAG_DOMAIN] != 0) {
-            // name the first such particle: its type, position and velocity
-            const int i = c->hflags[FLAG_DOMAIN_FIRST];
-            double v[5] = {0, 0, 0, 0, 0};
-            int32_t t = 0;
-            const int fs[5] = {VOX_F_X, VOX_F_Y, VOX_F_Z, VOX_F_U, VOX_F_V};
-            if (i >= 0 && i < c->tp) {
-                for (int k = 0; k < 5; ++k)
-                    VOX_HIP(hipMemcpy(&v[k], c->f[fs[k]] + i, sizeof(double), hipMemcpyDeviceToHost));
-                VOX_HIP(hipMemcpy(&t, c->ptype + i, sizeof(int32_t), hipMemcpyDeviceToHost));
-            }
-            char buf[256];
-            std::snprintf(buf, sizeof buf, "; first: particle %d (PTYPE %d) at (%.6g, %.6g, %.6g), velocity (%.6g, %.6g)",
-                          i, t, v[0], v[1], v[2], v[3], v[4]);
-            return fail(VOX_ERR_DOMAIN, "particle outside the neighbour grid box (" +
-                                            std::to_string(c->hflags[FLAG_DOMAIN]) + " particles)" + buf);
-        }
         if (c->hflags[fo] == 0) return VOX_OK;
+        if (attempt == 3) break;
         int need = c->hflags[fm];
         int cap = ((need + need / 4) + 7) & ~7;
         VOX_TRY(ensure_list_storage(c, which, cap));
+        VOX_TRY(search_one(c, which, re));
+        VOX_TRY(read_flags(c));
     }
     return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+static int build_one(vox_ctx* c, int which, double re)
+{
+    VOX_TRY(launch_one(c, which, re));
+    VOX_TRY(read_flags(c));
+    if (c->hflags[FLAG_DOMAIN] != 0) return domain_error(c);
+    return regrow_one(c, which, re);
 }
 
 // Tile-shape autotuner of the neighbour search: times both walks (one thread
@@ -601,12 +637,44 @@ int autotune_lists(vox_ctx* c, int reps, int32_t* modes)
     return rc;
 }
 
+// List L on the side stream: the context's stream, scan scratch and
+// block-maximum slots are swapped for the side ones while its work is queued.
+static int launch_side(vox_ctx* c, int which, double re)
+{
+    std::swap(c->stream, c->stream2);
+    std::swap(c->scan_blocks, c->scan_blocks2);
+    std::swap(c->scan_blocks_cap, c->scan_blocks2_cap);
+    std::swap(c->blockmax, c->blockmax2);
+    const int rc = launch_one(c, which, re);
+    std::swap(c->stream, c->stream2);
+    std::swap(c->scan_blocks, c->scan_blocks2);
+    std::swap(c->scan_blocks_cap, c->scan_blocks2_cap);
+    std::swap(c->blockmax, c->blockmax2);
+    return rc;
+}
+
 int build_neighbors(vox_ctx* c)
 {
     VOX_TRY(fill_i32(c, c->dflags + FLAG_DOMAIN, 0, 1));
     VOX_TRY(fill_i32(c, c->dflags + FLAG_DOMAIN_FIRST, 0x7fffffff, 1));
-    VOX_TRY(build_one(c, 0, c->p.re_s));
-    if (c->two_lists) VOX_TRY(build_one(c, 1, c->p.re_l));
+    if (c->two_lists && c->stream2) {
+        // The two lists are independent: list L is built on a second stream
+        // beside list S (the side stream first waits for everything queued so
+        // far), and the main stream waits for it before the one flag read.
+        VOX_HIP(hipEventRecord(c->ev_fork, c->stream));
+        VOX_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        VOX_TRY(launch_one(c, 0, c->p.re_s));
+        VOX_TRY(launch_side(c, 1, c->p.re_l));
+        VOX_HIP(hipEventRecord(c->ev_join, c->stream2));
+        VOX_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        VOX_TRY(read_flags(c));
+        if (c->hflags[FLAG_DOMAIN] != 0) return domain_error(c);
+        VOX_TRY(regrow_one(c, 0, c->p.re_s));
+        VOX_TRY(regrow_one(c, 1, c->p.re_l));
+    } else {
+        VOX_TRY(build_one(c, 0, c->p.re_s));
+        if (c->two_lists) VOX_TRY(build_one(c, 1, c->p.re_l));
+    }
     c->st.max_count_s = c->hflags[FLAG_MAXCNT_S];
     c->st.max_count_l = c->two_lists ? c->hflags[FLAG_MAXCNT_L] : c->hflags[FLAG_MAXCNT_S];
     return VOX_OK;
