@@ -22,7 +22,7 @@
 // start) -> atomic scatter -> per-cell rank sort by id (one wave per cell,
 // LDS tile) with position records copied into cell order -> per-particle stencil
 // walk (one thread per particle, or one wave per cell) writing sliced rows
-// (sl_index).
+// (sl_index). With two radii the two lists' pipelines run on two streams.
 #include <utility>
 
 #include "vox_internal.hpp"
