@@ -410,18 +410,19 @@ def test_gpu_runs_are_deterministic(hip):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("caps", [(4, 8), (0, 8), (4, 0)], ids=["both", "list-L", "list-S"])
 @pytest.mark.parametrize("dim,mode", [(2, 1), (2, 2), (3, 2)])
-def test_row_capacity_grows(hip, oracle, dim, mode):
+def test_row_capacity_grows(hip, oracle, dim, mode, caps):
     """Rows that overflow a too-small capacity are detected, storage grows and
-    the search repeats, in both search kernels."""
+    the search repeats, in both search kernels, for either list alone or both
+    (the two lists are built on two streams)."""
     if dim == 2:
         case = vx.jitter(vx.dam_break_2d_c0(), 0.001, seed=2)
         p = vx.dam2d_params(case)
     else:
         case = vx.jitter(vx.dam_break(3, 0.005, (16, 16, 12), (48, 20, 13)), 0.0005, seed=3)
         p = vx.dam3d_params(case)
-    p.cap_s = 4
-    p.cap_l = 8
+    p.cap_s, p.cap_l = caps
     p.list_mode = mode
     g, c = _pair(case, p, hip, oracle)
     g.NEIGHBOR()
