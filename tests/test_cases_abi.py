@@ -201,3 +201,16 @@ def test_reference_input_files_parse(tmp_path):
     assert c.tp == 1122 and (c.gp, c.wp, c.fp) == (162, 312, 648)
     c3 = vx.cases.read_grd(os.path.join(ref, "input-dam3d.grd"), 3)
     assert c3.tp == 8140 and (c3.gp, c3.wp, c3.fp) == (2732, 4384, 1024)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py runs on one GPU; asked for more it says so and exits 2
+    before touching a device."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
