@@ -260,6 +260,31 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
     }
 }
 
+// The same order with one thread per particle, for grids whose cells hold a
+// few particles each (a wave per cell would leave most lanes idle and visit
+// every empty cell): the particle at scattered slot t counts the ids of its
+// cell's members below its own, which is its rank in the ascending run.
+__global__ __launch_bounds__(kBlock) void k_cell_rank(
+    int n, int dim, const int32_t* __restrict__ cell, const int32_t* __restrict__ start,
+    const int32_t* __restrict__ order_raw, PosRec* __restrict__ rec,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int32_t id = order_raw[t];
+    const int32_t c = cell[id];
+    const int s = start[c], e = start[c + 1];
+    int rank = 0;
+    for (int m = s; m < e; ++m) rank += (order_raw[m] < id);
+    PosRec rr;
+    rr.x = x[id];
+    rr.y = y[id];
+    rr.z = (dim == 3) ? z[id] : 0.0;
+    rr.id = id;
+    rr.aux = 0;
+    rec[s + rank] = rr;
+}
+
 // Per-particle stencil walk. R is the accept radius (3D and the CUDA path:
 // re+DELTA, inclusive; 2D: re, exclusive).
 __device__ __forceinline__ int wave_max_i32(int v)
@@ -476,6 +501,16 @@ int ensure_list_storage(vox_ctx* c, int which, int cap)
 
 static int search_one(vox_ctx* c, int which, double re);
 
+// Expected particles per cell (cell volume / DL^dim) below 24: the cells are
+// sorted one thread per particle rather than one wave per cell.
+static bool few_per_cell(const vox_ctx* c, const DevList& L)
+{
+    const double per_cell = (c->p.dim == 3)
+        ? L.g.hx * L.g.hx * L.g.hz / (c->p.dl * c->p.dl * c->p.dl)
+        : L.g.hx * L.g.hx / (c->p.dl * c->p.dl);
+    return per_cell < 24.0;
+}
+
 // Grid, cell sort and one search pass of list `which` on c->stream, without
 // waiting for it. Reports, once the stream has run, go to the list's overflow
 // and longest-row flags and to the shared domain flags.
@@ -509,10 +544,17 @@ static int launch_one(vox_ctx* c, int which, double re)
         VOX_TRY(fill_i32(c, L.cell_fill, 0, L.g.tnc));
         k_scatter<<<grid_for(n), kBlock, 0, c->stream>>>(n, L.cell, L.cell_start, L.cell_fill,
                                                            L.order_raw);
-        long nb = (L.g.tnc + (kBlock / kWave) - 1) / (kBlock / kWave);
-        k_cell_sort<<<(int)nb, kBlock, 0, c->stream>>>(L.g.tnc, c->p.dim, L.cell_start,
-                                                        L.order_raw, L.rec, c->f[VOX_F_X],
-                                                        c->f[VOX_F_Y], c->f[VOX_F_Z]);
+        // cells of fewer than ~24 expected particles: one thread per particle
+        if (few_per_cell(c, L)) {
+            k_cell_rank<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->p.dim, L.cell, L.cell_start,
+                                                                L.order_raw, L.rec, c->f[VOX_F_X],
+                                                                c->f[VOX_F_Y], c->f[VOX_F_Z]);
+        } else {
+            long nb = (L.g.tnc + (kBlock / kWave) - 1) / (kBlock / kWave);
+            k_cell_sort<<<(int)nb, kBlock, 0, c->stream>>>(L.g.tnc, c->p.dim, L.cell_start,
+                                                            L.order_raw, L.rec, c->f[VOX_F_X],
+                                                            c->f[VOX_F_Y], c->f[VOX_F_Z]);
+        }
         VOX_HIP(hipGetLastError());
     }
     return search_one(c, which, re);
