@@ -334,6 +334,10 @@ int vox_attach_lpc(vox_ctx* ctx, vox_lpc* lpc);
  * network and coupling setup of a learned solve), lpc_apply (the
  * preconditioner inside flexible CG). */
 int vox_prof_enable(vox_ctx* ctx, int on);
+/* Restrict the events to one class (NULL: every class), so that a timed run
+ * pays for the events of the kernel it measures only. No counterpart in the
+ * reference (it has no profiling). */
+int vox_prof_only(vox_ctx* ctx, const char* kernel);
 int vox_prof_get(vox_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int vox_prof_reset(vox_ctx* ctx);
 

@@ -139,6 +139,7 @@ PROTOTYPES = {
     "vox_get_cells": (C.c_int, [_P, C.c_int, _i32p]),
     "vox_ppe_export": (C.c_int, [_P, _i32p, _i32p, _i32p, _i32p, _dp, _dp, _i32p]),
     "vox_prof_enable": (C.c_int, [_P, C.c_int]),
+    "vox_prof_only": (C.c_int, [_P, C.c_char_p]),
     "vox_prof_get": (C.c_int, [_P, C.c_char_p, _dp, _i64p]),
     "vox_prof_reset": (C.c_int, [_P]),
     "vox_grd_count": (C.c_int, [C.c_char_p, _ip]),

@@ -703,6 +703,16 @@ int vox_prof_enable(vox_ctx* c, int on)
     return VOX_OK;
 }
 
+int vox_prof_only(vox_ctx* c, const char* kernel)
+{
+    VOX_TRY(check_ctx(c));
+    if (!kernel) { c->prof_only = -1; return VOX_OK; }
+    int k = kclass_from_name(kernel);
+    if (k < 0) return fail(VOX_ERR_ARG, std::string("unknown kernel class ") + kernel);
+    c->prof_only = k;
+    return VOX_OK;
+}
+
 int vox_prof_get(vox_ctx* c, const char* kernel, double* total_ms, int64_t* launches)
 {
     VOX_TRY(check_ctx(c));

@@ -306,6 +306,7 @@ struct vox_ctx {
 
     // profiling
     bool prof_on = false;
+    int prof_only = -1;            // profile this kernel class only (-1: every class)
     std::vector<vox::ProfRec> prof_pending;
     std::vector<hipEvent_t> prof_free;
     double prof_ms[32] = {};
@@ -330,8 +331,13 @@ int prof_drain(vox_ctx* c);
 
 struct ProfScope {
     vox_ctx* c; int k; hipEvent_t a = nullptr;
-    ProfScope(vox_ctx* cc, int kk) : c(cc), k(kk) { if (c->prof_on) prof_begin(c, k, &a); }
-    ~ProfScope() { if (c->prof_on) prof_end(c, k, a); }
+    bool on;
+    ProfScope(vox_ctx* cc, int kk)
+        : c(cc), k(kk), on(cc->prof_on && (cc->prof_only < 0 || cc->prof_only == kk))
+    {
+        if (on) prof_begin(c, k, &a);
+    }
+    ~ProfScope() { if (on) prof_end(c, k, a); }
 };
 
 inline ListView view(const vox_ctx* c, int which)
