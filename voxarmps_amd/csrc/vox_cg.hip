@@ -510,16 +510,17 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
         k_cg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
         VOX_HIP(hipGetLastError());
     }
-    // Check schedule: the first batch runs to two iterations short of the last
-    // solve's count (consecutive solves need similar counts), then the flag is
-    // read after every cg_check iterations (default 1), so no kernel is queued
-    // after convergence.
+    // Check schedule: the first batch runs the last solve's count (consecutive
+    // solves need similar counts), then the flag is read after every cg_check
+    // iterations (default 1). The convergence test is on the device and kernels
+    // queued past it return at entry, so the host reads the flags about once per
+    // solve; the setup's partial sums (nnz) are read with them.
     const int check = c->p.cg_check > 0 ? c->p.cg_check : 1;
     int& expect = std::is_same<VT, double>::value ? c->cg_expect : c->cg_expect_inner;
-    const int first = expect - 2;
+    const int first = expect;
     int it = 0;
-    VOX_TRY(read_flags(c));
-    c->nnz_total = (long)c->hscal[S_NNZ];
+    bool read = false;
+    c->hflags[FLAG_CG_DONE] = 0;        // host copy; the device decides
     while (!c->hflags[FLAG_CG_DONE] && it < a.maxit) {
         const int batch = (it == 0 && first > check) ? first : check;
         for (int k = 0; k < batch && it < a.maxit; ++k) {
@@ -539,7 +540,10 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
         }
         VOX_HIP(hipGetLastError());
         VOX_TRY(read_flags(c));
+        read = true;
     }
+    if (!read) VOX_TRY(read_flags(c));
+    c->nnz_total = (long)c->hscal[S_NNZ];
     expect = c->hflags[FLAG_CG_ITERS];
     *iters_out = c->hflags[FLAG_CG_ITERS];
     return VOX_OK;
@@ -564,10 +568,10 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
     }
     const int check = c->p.cg_check > 0 ? c->p.cg_check : 1;
     int& expect = std::is_same<VT, double>::value ? c->fcg_expect : c->fcg_expect_inner;
-    const int first = expect - 2;
+    const int first = expect;
     int it = 0;
-    VOX_TRY(read_flags(c));
-    c->nnz_total = (long)c->hscal[S_NNZ];
+    bool read = false;
+    c->hflags[FLAG_CG_DONE] = 0;        // host copy; the device decides
     while (!c->hflags[FLAG_CG_DONE] && it < a.maxit) {
         const int batch = (it == 0 && first > check) ? first : check;
         for (int k = 0; k < batch && it < a.maxit; ++k) {
@@ -591,7 +595,10 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
         }
         VOX_HIP(hipGetLastError());
         VOX_TRY(read_flags(c));
+        read = true;
     }
+    if (!read) VOX_TRY(read_flags(c));
+    c->nnz_total = (long)c->hscal[S_NNZ];
     expect = c->hflags[FLAG_CG_ITERS];
     *iters_out = c->hflags[FLAG_CG_ITERS];
     return VOX_OK;
