@@ -141,6 +141,14 @@ def test_lpc_entry_points_check_arguments():
     assert be.lib.vox_lpc_destroy(None) == 0
 
 
+def test_profiling_entry_points_check_arguments():
+    """Profiling calls refuse a null context before any device work."""
+    be = vx.hip_backend()
+    assert be.lib.vox_prof_only(None, b"spmv") != 0
+    assert be.lib.vox_prof_enable(None, 1) != 0
+    assert be.lib.vox_prof_reset(None) != 0
+
+
 def test_grd_round_trip_through_c_reader(tmp_path):
     be = vx.hip_backend()
     for c in (vx.jitter(vx.dam_break_2d_c0(), 0.001),
