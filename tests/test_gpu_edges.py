@@ -90,3 +90,29 @@ def test_volume_fraction_single_and_two_phase(hip, oracle):
     assert np.array_equal(cg, cc) and np.count_nonzero(cg) > 0
     g.close()
     c.close()
+
+
+def test_profiling_one_class(hip):
+    """vox_prof_only limits the events to one kernel class; NULL brings every
+    class back; an unknown class name is refused."""
+    import ctypes as C
+    case = vx.dam_break_2d_c0()
+    p = vx.dam2d_params(case)
+    lib = hip.lib
+
+    def launches(s, name):
+        t, n = C.c_double(0.0), C.c_int64(0)
+        assert lib.vox_prof_get(s.ctx, name, C.byref(t), C.byref(n)) == 0
+        return n.value
+
+    with vx.Simulation(p, case, backend=hip) as s:
+        s.init()
+        assert lib.vox_prof_only(s.ctx, b"no-such-class") != 0
+        for only, expect_pred in ((b"spmv", False), (None, True)):
+            assert lib.vox_prof_reset(s.ctx) == 0
+            assert lib.vox_prof_only(s.ctx, only) == 0
+            assert lib.vox_prof_enable(s.ctx, 1) == 0
+            s.step(2)
+            assert lib.vox_prof_enable(s.ctx, 0) == 0
+            assert launches(s, b"spmv") > 0
+            assert (launches(s, b"pred") > 0) == expect_pred
