@@ -222,3 +222,27 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/voxarmps.h compiles as C99 and links against the library: a C
+    caller (the reference's C++ loop, or an FFI) needs no C++ types."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler on this machine")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "caller.c"
+    src.write_text('#include "voxarmps.h"\n'
+                   "int main(void) {\n"
+                   "    vox_params p;\n"
+                   "    if (vox_params_default(&p, 3, 0.005) != VOX_OK) return 2;\n"
+                   "    return (p.dim == 3 && p.re_s > 0.0) ? 0 : 1;\n"
+                   "}\n")
+    exe = tmp_path / "caller"
+    libdir = os.path.join(root, "voxarmps_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(root, "include"),
+                        str(src), "-L", libdir, "-lvoxarmps", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert subprocess.run([str(exe)], timeout=60).returncode == 0
