@@ -40,37 +40,64 @@ int st_density(vox_ctx* c)
 }
 
 // ---- PNUM / PNUMSTAR: functions.cu:94-123, 157-180 -------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_pnd(int n, int dim, int ktype, double re,
-                                                double ncorr, ListView L,
-                                                const double* __restrict__ x,
-                                                const double* __restrict__ y,
-                                                const double* __restrict__ z,
+// One 32-byte record per particle (x, y, z; z = 0 in 2D), so a neighbour costs
+// one gather.
+__global__ void k_pack_pos(int n, int dim, const double* __restrict__ x,
+                           const double* __restrict__ y, const double* __restrict__ z,
+                           PosRec* __restrict__ rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PosRec r;
+    r.x = x[i]; r.y = y[i]; r.z = (dim == 3) ? z[i] : 0.0;
+    r.id = i; r.aux = 0;
+    rec[i] = r;
+}
+
+// n_i = sum over the row of W(d_ij), j != i, in row order. Branch-free per
+// entry with 8 record loads in flight; the 3-term distance equals the 2D one
+// bit for bit (dz = 0).
+__global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, double ncorr,
+                                                ListView L, const PosRec* __restrict__ rec,
                                                 double* __restrict__ out,
                                                 const double* __restrict__ p,
                                                 double* __restrict__ pnew)
 {
-    int i = xcd_tid();
+    const int i = xcd_tid();
     if (i >= n) return;
-    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    const PosRec me = rec[i];
     double sum = 0.0;
     const double inv_re = 1.0 / re;
-    int cnt = L.cnt[i];
+    const int cnt = L.cnt[i];
     const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
     for (int g = 0; (g << 3) < cnt; ++g) {
         int32_t jc[8];
         load_chunk8(rowp, g, jc);
+        PosRec rr[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if ((g << 3) + e >= cnt) break;
-            int j = jc[e];
-            if (j == i) continue;
-            double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
-            sum = sum + wfun_r(d, ktype, re, inv_re);
+        for (int u = 0; u < 8; ++u) jc[u] = ((g << 3) + u < cnt) ? jc[u] : i;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rr[u] = rec[jc[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double dx = rr[u].x - me.x, dy = rr[u].y - me.y, dz = rr[u].z - me.z;
+            const double w = wfun_r(sqrt(dx * dx + dy * dy + dz * dz), ktype, re, inv_re);
+            sum = (jc[u] != i) ? sum + w : sum;
         }
     }
     if (ktype != 2) sum = sum * ncorr;
     out[i] = sum;
     if (pnew) pnew[i] = p[i];
+}
+
+static void launch_pnd(vox_ctx* c, const DevList& L, double re, double ncorr, const double* x,
+                       const double* y, const double* z, double* out, const double* p,
+                       double* pnew)
+{
+    k_pack_pos<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.dim, x, y, z, c->pos_rec);
+    k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.ktype, re, ncorr,
+                                                      ListView{L.cnt, L.ids, L.cap}, c->pos_rec,
+                                                      out, p, pnew);
 }
 
 int st_pnd(vox_ctx* c, int on_star, int loopstarted)
@@ -82,9 +109,8 @@ int st_pnd(vox_ctx* c, int on_star, int loopstarted)
     double* out = c->f[on_star ? VOX_F_NSTAR : VOX_F_N];
     // PNUMSTAR applies no Ncorrection: multiplying by 1.0 is exact
     double ncorr = on_star ? 1.0 : c->p.ncorrection;
-    k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
-        c->tp, c->p.dim, c->p.ktype, c->p.re_s, ncorr, view(c, VOX_LIST_S), x, y, z, out,
-        c->f[VOX_F_P], (!on_star && !loopstarted) ? c->f[VOX_F_PNEW] : nullptr);
+    launch_pnd(c, list_of(c, VOX_LIST_S), c->p.re_s, ncorr, x, y, z, out, c->f[VOX_F_P],
+               (!on_star && !loopstarted) ? c->f[VOX_F_PNEW] : nullptr);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
@@ -100,9 +126,8 @@ int st_n0(vox_ctx* c)
     for (double v : h) if (v > mx) mx = v;
     c->n0s = mx;
     if (c->two_lists) {
-        k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
-            c->tp, c->p.dim, c->p.ktype, c->p.re_l, c->p.ncorrection, view(c, VOX_LIST_L),
-            c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], c->scratch_d, nullptr, nullptr);
+        launch_pnd(c, list_of(c, VOX_LIST_L), c->p.re_l, c->p.ncorrection, c->f[VOX_F_X],
+                   c->f[VOX_F_Y], c->f[VOX_F_Z], c->scratch_d, nullptr, nullptr);
         VOX_HIP(hipGetLastError());
         VOX_HIP(hipMemcpyAsync(h.data(), c->scratch_d, sizeof(double) * c->tp,
                                hipMemcpyDeviceToHost, c->stream));
