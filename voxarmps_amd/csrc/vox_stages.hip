@@ -611,15 +611,33 @@ struct GradArgs {
     ListView L;
     const double *x, *y, *z, *pn, *ph, *rho, *us, *vs, *ws;
     double *un, *vn, *wn;
+    struct GradRec* rec;
 };
 
+// One 64-byte record per particle with every field the gradient gathers
+// (x*, y*, z* with z = 0 in 2D, p, p^).
+struct __align__(64) GradRec { double x, y, z, pn, ph, pad0, pad1, pad2; };
+
+__global__ void k_pack_grad(GradArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    GradRec r;
+    r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
+    r.pn = a.pn[i]; r.ph = a.ph[i];
+    r.pad0 = r.pad1 = r.pad2 = 0.0;
+    a.rec[i] = r;
+}
+
+// Branch-free per entry with 8 records in flight; the 3-term distance equals
+// the 2D one bit for bit, and the z sum is only read in 3D.
 __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
 {
     int i = a.first + xcd_tid();
     if (i >= a.n) return;
     const int dim = a.dim;
-    double xi = a.x[i], yi = a.y[i], zi = (dim == 3) ? a.z[i] : 0.0;
-    double pi_ = a.pn[i], phi_ = a.ph[i];
+    const GradRec me = a.rec[i];
+    const double xi = me.x, yi = me.y, zi = me.z, pi_ = me.pn, phi_ = me.ph;
     double s1 = 0, s2 = 0, s3 = 0;
     const double inv_re = 1.0 / a.re;
     int cnt = a.L.cnt[i];
@@ -627,24 +645,29 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
     for (int g = 0; (g << 3) < cnt; ++g) {
         int32_t jc[8];
         load_chunk8(rowp, g, jc);
+        double px[8], py[8], pz[8], pp[8], ph[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) jc[e] = ((g << 3) + e < cnt) ? jc[e] : i;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            if ((g << 3) + e >= cnt) break;
-            int j = jc[e];
-            if (j == i) continue;
-            double dx = a.x[j] - xi, dy = a.y[j] - yi, dz = 0.0, D;
-            if (dim == 3) { dz = a.z[j] - zi; D = sqrt(dx * dx + dy * dy + dz * dz); }
-            else D = sqrt(dx * dx + dy * dy);
-            double wv = wfun_r(D, a.ktype, a.re, inv_re);
+            const GradRec* r = a.rec + jc[e];
+            px[e] = r->x; py[e] = r->y; pz[e] = r->z; pp[e] = r->pn; ph[e] = r->ph;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const bool use = (jc[e] != i);
+            const double dx = px[e] - xi, dy = py[e] - yi, dz = pz[e] - zi;
+            const double D = sqrt(dx * dx + dy * dy + dz * dz);
+            const double wv = wfun_r(D, a.ktype, a.re, inv_re);
             // kh 1: p_j + p_i - (p^_i + p^_j) (functions.cu:275-278); kh 2, the
             // reference's CUDA kernel: p_j + p_i - 2 p^_i (382-385); kh 0: p_j - p^_i
-            double dp = (a.kh == 1)   ? (a.pn[j] + pi_ - (phi_ + a.ph[j]))
-                        : (a.kh == 2) ? (a.pn[j] + pi_ - 2.0 * phi_)
-                                      : (a.pn[j] - phi_);
+            const double dp = (a.kh == 1)   ? (pp[e] + pi_ - (phi_ + ph[e]))
+                              : (a.kh == 2) ? (pp[e] + pi_ - 2.0 * phi_)
+                                            : (pp[e] - phi_);
             const double inv_D = 1.0 / D;                 // x / D / D via div_by
-            s1 = s1 + div_by(div_by(dp * dx * wv, D, inv_D), D, inv_D);
-            s2 = s2 + div_by(div_by(dp * dy * wv, D, inv_D), D, inv_D);
-            if (dim == 3) s3 = s3 + div_by(div_by(dp * dz * wv, D, inv_D), D, inv_D);
+            s1 = use ? s1 + div_by(div_by(dp * dx * wv, D, inv_D), D, inv_D) : s1;
+            s2 = use ? s2 + div_by(div_by(dp * dy * wv, D, inv_D), D, inv_D) : s2;
+            s3 = use ? s3 + div_by(div_by(dp * dz * wv, D, inv_D), D, inv_D) : s3;
         }
     }
     double g = a.relaxp * (a.gnum * a.dt / a.n0 / a.rho[i]);
@@ -678,8 +701,10 @@ int st_pressgrad(vox_ctx* c)
     a.pn = c->f[VOX_F_PNEW]; a.ph = c->f[VOX_F_PHAT]; a.rho = c->f[VOX_F_RHO];
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.un = c->f[VOX_F_UNEW]; a.vn = c->f[VOX_F_VNEW]; a.wn = c->f[VOX_F_WNEW];
+    a.rec = reinterpret_cast<GradRec*>(c->pred_rec);     // the predictor's 64-byte slots
     int m = c->tp - a.first;
     if (m <= 0) return VOX_OK;
+    k_pack_grad<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     k_pressgrad<<<grid_for(m), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
