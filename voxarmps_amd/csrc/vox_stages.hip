@@ -807,10 +807,11 @@ int st_euler(vox_ctx* c)
 // than margin in total during the pass; the replay accumulates each touched
 // particle's path length and raises a flag if any exceeds margin/2 — then an
 // undo log restores the state and the pass repeats with a wider screen.
-__global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double rc, ListView L,
-                                                        const double* __restrict__ x,
-                                                        const double* __restrict__ y,
-                                                        const double* __restrict__ z,
+// Positions come as packed 32-byte records (z = 0 in 2D: the 3-term distance
+// equals the 2D one bit for bit); 8 record loads are in flight per row chunk
+// and candidates are emitted in slot order.
+__global__ __launch_bounds__(kBlock) void k_coll_screen(int n, double rc, ListView L,
+                                                        const PosRec* __restrict__ rec,
                                                         const int32_t* __restrict__ off,
                                                         int32_t* __restrict__ cnt,
                                                         int32_t* __restrict__ ci,
@@ -818,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double r
 {
     int i = xcd_tid();
     if (i >= n) return;
-    double xi = x[i], yi = y[i], zi = (dim == 3) ? z[i] : 0.0;
+    const PosRec me = rec[i];
     long pos = off ? off[i] : 0;
     int k = 0;
     int rows = L.cnt[i];
@@ -826,15 +827,21 @@ __global__ __launch_bounds__(kBlock) void k_coll_screen(int n, int dim, double r
     for (int g = 0; (g << 3) < rows; ++g) {
         int32_t jc[8];
         load_chunk8(rowp, g, jc);
+        double px[8], py[8], pz[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) jc[e] = ((g << 3) + e < rows) ? jc[e] : i;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            if ((g << 3) + e >= rows) break;
-            int j = jc[e];
-            if (j == i) continue;
-            double d = pdist(dim, x, y, z, i, j, xi, yi, zi);
-            if (!(d < rc)) continue;
-            if (off && pos + k < cap) { ci[pos + k] = i; cj[pos + k] = j; }
-            ++k;
+            const PosRec* r = rec + jc[e];
+            px[e] = r->x; py[e] = r->y; pz[e] = r->z;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const double dx = px[e] - me.x, dy = py[e] - me.y, dz = pz[e] - me.z;
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            const bool hit = (jc[e] != i) && (d < rc);
+            if (hit && off && pos + k < cap) { ci[pos + k] = i; cj[pos + k] = jc[e]; }
+            k += hit ? 1 : 0;
         }
     }
     if (!off) cnt[i] = k;
@@ -1162,7 +1169,8 @@ int st_collision(vox_ctx* c)
     for (int attempt = 0; attempt < 8; ++attempt) {
         double margin = p.coll * c->coll_margin;
         double rc = p.coll + margin;
-        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, nullptr,
+        k_pack_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, X, Y, Z, c->pos_rec);
+        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, rc, L, c->pos_rec, nullptr,
                                                               c->coll_cnt, nullptr, nullptr, 0);
         VOX_TRY(scan_exclusive(c, c->coll_cnt, c->coll_off, n, c->coll_off + n));
         int32_t total = 0;
@@ -1172,7 +1180,7 @@ int st_collision(vox_ctx* c)
         c->st.collision_cand = total;
         if (total == 0) { c->st.collisions = 0; return VOX_OK; }
         VOX_TRY(coll_reserve(c, total));
-        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, rc, L, X, Y, Z, c->coll_off,
+        k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, rc, L, c->pos_rec, c->coll_off,
                                                               nullptr, c->coll_i, c->coll_j,
                                                               c->coll_cap);
         CollArgs a;
