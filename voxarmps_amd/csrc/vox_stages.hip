@@ -582,15 +582,19 @@ __global__ void k_phat(int n, ListView L, const double* __restrict__ pn, double*
     double mn = 999999999999999.0;
     int cnt = L.cnt[i];
     const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    // All 8 loads of a chunk are issued before the compares (padding slots
+    // load p_i and are dropped by the select); entries are compared in row
+    // order with a strict <, so ties keep the first value as before.
     for (int g = 0; (g << 3) < cnt; ++g) {
         int32_t jc[8];
+        double v[8];
         load_chunk8(rowp, g, jc);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if ((g << 3) + e >= cnt) break;
-            double v = pn[jc[e]];
-            if (v < mn) mn = v;
-        }
+        for (int e = 0; e < 8; ++e) jc[e] = ((g << 3) + e < cnt) ? jc[e] : i;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = pn[jc[e]];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) mn = ((g << 3) + e < cnt && v[e] < mn) ? v[e] : mn;
     }
     ph[i] = mn;
 }
