@@ -190,35 +190,65 @@ __device__ __forceinline__ bool cell_coords(const Grid& g, double x, double y, d
     return ic >= 1 && ic <= g.ncx && jc >= 1 && jc <= g.ncy && kc >= 1 && kc <= g.ncz;
 }
 
-__global__ void k_cell_keys(Grid g, int n, const double* __restrict__ x,
+// ctr[key] += 1 for every active lane of the wave, with one atomic per
+// distinct key: consecutive particles mostly share a cell, so per-lane atomics
+// on one counter serialise. Returns the lane's slot among the key's
+// increments (the counter's old value plus its rank among the wave's lanes
+// with that key). Must be reached by every lane of the wave; the loop runs
+// once per distinct key among the active lanes.
+__device__ __forceinline__ int32_t wave_add1(int32_t* __restrict__ ctr, int32_t key, bool active)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long todo = __ballot(active);
+    int32_t slot = 0;
+    while (todo) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const int32_t lk = __shfl(key, lead);
+        const bool mine = active && key == lk;
+        const unsigned long long m = __ballot(mine);
+        int32_t base = 0;
+        if (lane == lead) base = atomicAdd(&ctr[lk], (int32_t)__popcll(m));
+        base = __shfl(base, lead);
+        if (mine) slot = base + (int32_t)__popcll(m & ((1ull << lane) - 1ull));
+        todo &= ~m;
+    }
+    return slot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_keys(Grid g, int n, const double* __restrict__ x,
                             const double* __restrict__ y, const double* __restrict__ z,
                             int32_t* __restrict__ cell, int32_t* __restrict__ count,
                             int32_t* __restrict__ flags)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int ic, jc, kc;
-    double zi = (g.dim == 3) ? z[i] : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n;
     int32_t key = 0;
-    if (cell_coords(g, x[i], y[i], zi, ic, jc, kc)) {
-        key = (ic - 1) + (jc - 1) * g.ncx + (kc - 1) * g.ncx * g.ncy;   // 2826-2834, 0-based
-    } else {
-        atomicAdd(&flags[FLAG_DOMAIN], 1);
-        atomicMin(&flags[FLAG_DOMAIN_FIRST], i);
+    if (act) {
+        int ic, jc, kc;
+        double zi = (g.dim == 3) ? z[i] : 0.0;
+        if (cell_coords(g, x[i], y[i], zi, ic, jc, kc)) {
+            key = (ic - 1) + (jc - 1) * g.ncx + (kc - 1) * g.ncx * g.ncy;   // 2826-2834, 0-based
+        } else {
+            atomicAdd(&flags[FLAG_DOMAIN], 1);
+            atomicMin(&flags[FLAG_DOMAIN_FIRST], i);
+        }
+        cell[i] = key;
     }
-    cell[i] = key;
-    atomicAdd(&count[key], 1);
+    wave_add1(count, key, act);
 }
 
-__global__ void k_scatter(int n, const int32_t* __restrict__ cell,
+// Places each particle in its cell's run of order_raw; the order inside a run
+// is arbitrary here (the per-cell id sort or rank that follows makes it
+// ascending by id).
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, const int32_t* __restrict__ cell,
                           const int32_t* __restrict__ start, int32_t* __restrict__ fill,
                           int32_t* __restrict__ order_raw)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t k = cell[i];
-    int32_t pos = start[k] + atomicAdd(&fill[k], 1);
-    order_raw[pos] = i;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n;
+    const int32_t k = act ? cell[i] : 0;
+    const int32_t slot = wave_add1(fill, k, act);
+    if (act) order_raw[start[k] + slot] = i;
 }
 
 // One wave per cell: rank of each id among the cell's ids (ids are unique), so
