@@ -139,6 +139,13 @@ int orc_set_params(orc_ctx* c, const vox_params* p)
     return VOX_OK;
 }
 
+int orc_get_params(orc_ctx* c, vox_params* p)
+{
+    if (!c || !p) return VOX_ERR_ARG;
+    *p = c->p;
+    return VOX_OK;
+}
+
 int orc_upload(orc_ctx* c, int field, const void* host)
 {
     if (!c || !host || field < 0 || field >= VOX_F_COUNT) return VOX_ERR_ARG;

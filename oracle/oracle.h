@@ -26,6 +26,7 @@ typedef struct orc_ctx orc_ctx;
 int  orc_create(orc_ctx** ctx, const vox_params* p, int tp, int gp, int wp, int fp);
 void orc_destroy(orc_ctx* ctx);
 int  orc_set_params(orc_ctx* ctx, const vox_params* p);
+int  orc_get_params(orc_ctx* ctx, vox_params* p);
 int  orc_upload(orc_ctx* ctx, int field, const void* host);
 int  orc_download(orc_ctx* ctx, int field, void* host);
 

@@ -27,6 +27,7 @@ PROTOTYPES = {
     "orc_create": (C.c_int, [C.POINTER(_P), _pp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "orc_destroy": (None, [_P]),
     "orc_set_params": (C.c_int, [_P, _pp]),
+    "orc_get_params": (C.c_int, [_P, _pp]),
     "orc_upload": (C.c_int, [_P, C.c_int, _P]),
     "orc_download": (C.c_int, [_P, C.c_int, _P]),
     "orc_density": (C.c_int, [_P]),

@@ -279,6 +279,38 @@ def test_collision_replay_bit_exact(hip, oracle, dim, vscale):
     c.close()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_collision_stale_list_widened_screen(hip, oracle, dim):
+    """COLLISION2 runs on list S built at the step's earlier positions
+    (main.cu:1050 vs 1306). Here the list is built, the particles are moved by
+    up to 0.5 l0 and given velocities large enough that the screen widens
+    past the list radius: the replay then takes every listed pair as a
+    candidate and still equals the sequential reference loop bit for bit."""
+    case, p, rng = _crowded_case(31 + dim, n=500, dim=dim)
+    p = vx.two_radius(p)
+    p.dt = 1e-4
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    l0 = case.l0
+    moved = {n: getattr(case, n.lower()) + rng.uniform(-0.5, 0.5, case.tp) * l0
+             for n in (("X", "Y", "Z") if dim == 3 else ("X", "Y"))}
+    vel = {n: rng.normal(0, 400.0, case.tp) for n in ("UNEW", "VNEW", "WNEW")}
+    for sim in (g, c):
+        for n, v in {**moved, **vel}.items():
+            sim.set(n, v)
+        sim.COLLISION2()
+    _assert_same(g, c, FLOAT_OUT["COLLISION2"], f"collision stale list d{dim}")
+    cnt, ids = c.neighbors(vx.LIST_S)
+    listed = int(cnt.sum()) - (case.tp if dim == 3 else 0)      # 3D rows list the particle itself
+    sg = g.status()
+    print(f"d{dim}: collisions {sg.collisions}, candidates {sg.collision_cand} of {listed} listed")
+    assert sg.collisions == c.status().collisions > 10
+    assert sg.collision_cand == listed                           # the widened screen took every pair
+    g.close()
+    c.close()
+
+
 def _sparse_pairs_case(seed, dim, side):
     """A lattice at l0 with U(-0.36 l0, 0.36 l0) jitter: scattered close pairs,
     so the collision candidates form many small components."""
@@ -333,6 +365,34 @@ def test_trajectory_2d(hip, oracle, nsteps):
         scale = max(np.max(np.abs(b)), 1.0)
         assert np.max(np.abs(a - b)) <= 1e-6 * scale, n
     _assert_lists(g, c, "trajectory end")
+    g.close()
+    c.close()
+
+
+def test_dtcalculation_steps(hip, oracle):
+    """dt_mode = 1: every step starts with DTcalculation (main.cu:1030,
+    functions.cu:1684-1698). The run starts from a DT above the Courant
+    limit, so the first step lowers it to COURANT DL / max(c01, c02); the
+    GPU and the oracle hold the same DT and simulated time, and the fields
+    agree as in the fixed-step trajectory."""
+    case = vx.jitter(vx.dam_break_2d_c0(), 0.05 * 0.01, seed=6)
+    p = vx.dam2d_params(case, tol=1e-10)
+    p.dt_mode, p.courant, p.c01, p.c02, p.dt, p.dt_max = 1, 0.04, 15.0, 25.0, 1e-3, 1.0
+    courant = 0.04 * p.dl / 25.0
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    g.step(12)
+    c.step(12)
+    pg, pc = g.get_params(), c.get_params()
+    assert pg.dt == pc.dt == courant
+    assert g.status().time == c.status().time
+    assert abs(g.status().time - 12 * courant) <= 1e-15
+    for n in ("X", "Y", "U", "V", "P"):
+        a, b = g.get(n), c.get(n)
+        scale = max(np.max(np.abs(b)), 1.0)
+        assert np.max(np.abs(a - b)) <= 1e-6 * scale, n
+    _assert_lists(g, c, "dt_mode 1")
     g.close()
     c.close()
 

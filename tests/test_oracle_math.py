@@ -732,3 +732,341 @@ def test_ref_gpu_is_2d_only(oracle):
     p.ref_gpu = 1
     with pytest.raises(RuntimeError):
         vx.Simulation(p, case, backend=oracle)
+
+
+# ---- PHATCALC (functions.cu:207-224) ------------------------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_phat_self_rule_and_axis_minimum(oracle, dim):
+    """p^_I = min over row I, started from 1e15. With p = 5 + r^2/l0^2 the
+    centre's own value is the smallest: 2D rows exclude the particle
+    (functions.cu:2907), so p^ = 6 (a nearest neighbour); 3D rows list it
+    (2886), so p^ = 5. With p = 3 x/l0 the farthest neighbour along -x of a
+    2.1 l0 row sits 2 spacings away (2 < 2.1 < 3): p^ = -6."""
+    case, p = _fluid_lattice(dim, half=5)
+    c = centre_index(case)
+    l0 = case.l0
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("PNEW", 5.0 + (case.x ** 2 + case.y ** 2 + case.z ** 2) / l0 ** 2)
+        s.PHATCALC()
+        ph_r2 = s.get("PHAT")[c]
+        s.set("PNEW", 3.0 * case.x / l0)
+        s.PHATCALC()
+        ph_x = s.get("PHAT")[c]
+    assert ph_r2 == (6.0 if dim == 2 else 5.0)
+    assert ph_x == -6.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_phat_is_minimum_over_bruteforce_row(oracle, dim):
+    """Random p on a jittered dam break: p^ of every particle equals the
+    minimum of p over its row from the independent O(N^2) search at re_s
+    (self included in 3D only), and 999999999999999 for an empty row."""
+    if dim == 2:
+        case = vx.jitter(vx.dam_break_2d_c0(), 0.1 * 0.01, seed=2)
+        p = vx.dam2d_params(case)
+    else:
+        case = vx.jitter(vx.dam_break(3, 0.005, (10, 10, 8), (30, 14, 10)), 0.1 * 0.005, seed=2)
+        p = vx.dam3d_params(case)
+    rng = np.random.default_rng(11)
+    pn = rng.uniform(-500.0, 3000.0, case.tp)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.NEIGHBOR()
+        s.set("PNEW", pn)
+        s.PHATCALC()
+        ph = s.get("PHAT")
+    cnt, ids = bruteforce_neighbors(p, p.re_s, case.x, case.y, case.z, 128)
+    want = np.array([pn[ids[i, :cnt[i]]].min() if cnt[i] else 999999999999999.0
+                     for i in range(case.tp)])
+    assert np.array_equal(ph, want)
+    # the particle's own p is the row minimum for a share of the rows in 3D only
+    own = ph == pn
+    assert (own.sum() > 0) == (dim == 3)
+
+
+def test_phat_empty_row_2d(oracle):
+    l0 = 0.01
+    x = np.array([0.05, 0.05 + 0.5 * l0, 0.25])
+    y = np.array([0.1, 0.1, 0.1])
+    case = vx.Case(2, l0, x, y, np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3),
+                   np.ones(3, np.int32))
+    p = vx.default_params(2, l0)
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 0.3, 0.0, 0.3
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.NEIGHBOR()
+        s.set("PNEW", np.array([4.0, 9.0, 1.0]))
+        s.PHATCALC()
+        ph = s.get("PHAT")
+    assert list(ph) == [9.0, 4.0, 999999999999999.0]
+
+
+# ---- SPS TURB (functions.cu:1796-1830) ----------------------------------------------
+
+def _sps_fields(case, a, b, c, e, f, h, x0, y0):
+    u = a * (case.y + y0) ** 2 + e * case.x
+    v = b * (case.x + x0) ** 2 + f * case.y
+    w = c * (case.x + x0) ** 2 + h * case.z
+    return u, v, w
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sps_turb_closed_form(oracle, dim):
+    """The MPS gradient (2/n0) sum (f_j - f_i) dx_m W/d^2 on a symmetric lattice
+    is (2/dim) df/dx_m for f quadratic. With u = a (y+y0)^2 + e x,
+    v = b (x+x0)^2 + f y (and w = c (x+x0)^2 + h z in 3D), g = 2/dim:
+    S11 = g e, S22 = g f, S33 = g h, S12 = g (a (y+y0) + b (x+x0)),
+    S13 = g c (x+x0), S23 = 0, all >= 0 near the centre, so NEUt =
+    (Cs DL)^2 (sqrt2 (S11+S22+S33) + 2 sqrt2 (S12+S13)) is linear with
+    dNEUt/dx = K g (b+c), dNEUt/dy = K g a (K = 2 sqrt2 (Cs DL)^2), and
+    TURB_k = sum_m 2 S_km g dNEUt/dx_m at the centre (x = y = z = 0)."""
+    case, p = _fluid_lattice(dim, half=6)
+    p.cs = 0.12
+    a, b, c, e, f, h, x0, y0 = 300.0, 170.0, 90.0, 40.0, 25.0, 60.0, 0.4, 0.3
+    if dim == 2:
+        c = h = 0.0
+    u, v, w = _sps_fields(case, a, b, c, e, f, h, x0, y0)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("UNEW", u)
+        s.set("VNEW", v)
+        s.set("WNEW", w)
+        s.SPS()
+        nt, t1, t2, t3 = (s.get(k) for k in ("NEUT", "TURB1", "TURB2", "TURB3"))
+    k = centre_index(case)
+    g = 2.0 / dim
+    Q = (p.cs * p.dl) ** 2
+    S11, S22, S33 = g * e, g * f, g * h
+    S12, S13 = g * (a * y0 + b * x0), g * c * x0
+    r2 = math.sqrt(2.0)
+    want_nt = Q * (r2 * (S11 + S22 + S33) + 2 * r2 * (S12 + S13))
+    K = 2 * r2 * Q
+    GNx, GNy = g * K * g * (b + c), g * K * g * a
+    want = [2 * S11 * GNx + 2 * S12 * GNy, 2 * S12 * GNx + 2 * S22 * GNy, 2 * S13 * GNx]
+    assert abs(nt[k] - want_nt) <= 1e-10 * want_nt
+    got = [t1[k], t2[k], t3[k] if dim == 3 else 0.0]
+    scale = max(abs(x) for x in want)
+    for gv, wv in zip(got, want):
+        assert abs(gv - wv) <= 1e-8 * scale
+    assert abs(want[0] - want[1]) > 0.1 * scale      # the terms are told apart
+
+
+# ---- MATRIX default source (functions.cu:990-1001) ----------------------------------
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_matrix_source_default_coefficient(oracle, dim):
+    """source_i = (n_i - n0)/(n0 dt^2) from the PND at x (not n*): with
+    n = (1 + eps) n0 and n* = n0 (every particle an unknown) the source is
+    eps/dt^2, and it is the right-hand side of the assembled system."""
+    case = lattice(dim, 0.01, 4)
+    p = params_for(case, 2.1, 4.0, dt=2e-4)
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(-0.2, 0.2, case.tp)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        n0s, n0l = s.n0()
+        for t, f in (("XSTAR", "X"), ("YSTAR", "Y"), ("ZSTAR", "Z")):
+            s.set(t, s.get(f))
+        s.set("NSTAR", np.full(case.tp, n0s))
+        s.set("N", n0s * (1.0 + eps))
+        s.BCON()
+        assert np.all(s.get("BCON") == 0)
+        s.MATRIX()
+        src = s.get("SOURCE")
+        A = s.ppe()
+    assert n0l != n0s
+    assert np.allclose(src * p.dt ** 2, eps, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(A["ids"], np.arange(case.tp))
+    assert np.array_equal(A["rhs"], src)
+
+
+# ---- PREDICTION (functions.cu:2283-2369): gravity, explicit pressure, TURB, NEUt ----
+
+@pytest.mark.parametrize("dim,coef2d", [(2, 1), (3, 1), (3, 0)])
+def test_prediction_all_terms_closed_form(oracle, dim, coef2d):
+    """u = y^2, p = p0 + a x + b y + c z, p^ constant, NEUt and TURB constant:
+    with the exact lattice lambda, u* - u = lapc dt (nu + NEUt)/dim + g dt +
+    dt TURB - (1 - relaxp) grdc dt (grad p)/(dim rho), where lapc, grdc are
+    the reference's literal 4 and 2 (coef2d) or 2 dim and dim; x* = x + dt u*."""
+    case, p = _fluid_lattice(dim)
+    p.coef2d = coef2d
+    p.lambda_ = lambda_closed_form(dim, 4.0) * case.l0 ** 2
+    p.gx, p.gy, p.gz = 0.3, -9.81, (0.7 if dim == 3 else 0.0)
+    a, b, cz = 2000.0, -3000.0, (1500.0 if dim == 3 else 0.0)
+    nut, T = 3e-5, (0.4, -0.25, 0.6)
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("U", case.y ** 2)
+        s.set("V", np.zeros(case.tp))
+        s.set("W", np.zeros(case.tp))
+        s.set("PNEW", 7000.0 + a * case.x + b * case.y + cz * case.z)
+        s.set("PHAT", np.full(case.tp, 123.0))
+        s.VISCOSITY()
+        s.set("NEUT", np.full(case.tp, nut))
+        for k, t in zip(("TURB1", "TURB2", "TURB3"), T):
+            s.set(k, np.full(case.tp, t))
+        s.PREDICTION()
+        us, vs, ws = s.get("USTAR"), s.get("VSTAR"), s.get("WSTAR")
+        xs, ys = s.get("XSTAR"), s.get("YSTAR")
+    k = centre_index(case)
+    lapc = 4.0 if coef2d else 2.0 * dim
+    grdc = 2.0 if coef2d else float(dim)
+    gc = (1 - p.relaxp) * grdc * p.dt / (dim * p.rho1)
+    want_u = lapc * p.dt * (p.nu1 + nut) / dim + p.gx * p.dt + p.dt * T[0] - gc * a
+    want_v = p.gy * p.dt + p.dt * T[1] - gc * b
+    assert abs(us[k] - want_u) <= 1e-9 * abs(want_u)
+    assert abs(vs[k] - want_v) <= 1e-9 * abs(want_v)
+    if dim == 3:
+        want_w = p.gz * p.dt + p.dt * T[2] - gc * cz
+        assert abs(ws[k] - want_w) <= 1e-9 * abs(want_w)
+    assert xs[k] == case.x[k] + p.dt * us[k] and ys[k] == case.y[k] + p.dt * vs[k]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("centre_type", [1, 2])
+def test_prediction_two_fluid_harmonic_viscosity(oracle, dim, centre_type):
+    """Fluid 2 on one side of the plane y = 0, u = y^2: neighbours of the
+    particle's own type use NEU = MEU_I/rho_I, the others the harmonic mean
+    2 MEU_I MEU_J/(MEU_I + MEU_J)/rho_I (functions.cu:2328-2337). Each side
+    holds half of sum W dy^2 = lambda n0/dim, so u* - u = lapc dt/(2 dim) *
+    (nu_I + H/rho_I), H = 2 m1 m2/(m1 + m2), m_k = nu_k rho_k."""
+    case, p = _fluid_lattice(dim)
+    p.coef2d = 0
+    p.lambda_ = lambda_closed_form(dim, 4.0) * case.l0 ** 2
+    p.rho2, p.nu2 = 800.0, 5e-5
+    if centre_type == 1:
+        case.ptype[case.y > 1e-9] = 2
+    else:
+        case.ptype[case.y > -1e-9] = 2
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("U", case.y ** 2)
+        s.set("V", np.zeros(case.tp))
+        s.set("W", np.zeros(case.tp))
+        s.set("PNEW", np.zeros(case.tp))
+        s.set("PHAT", np.zeros(case.tp))
+        s.VISCOSITY()
+        s.PREDICTION()
+        us = s.get("USTAR")
+    k = centre_index(case)
+    assert case.ptype[k] == centre_type
+    m1, m2 = p.nu1 * p.rho1, p.nu2 * p.rho2
+    H = 2 * m1 * m2 / (m1 + m2)
+    nu_i, rho_i = (p.nu1, p.rho1) if centre_type == 1 else (p.nu2, p.rho2)
+    want = 2.0 * dim * p.dt / (2 * dim) * (nu_i + H / rho_i)
+    assert abs(us[k] - want) <= 1e-9 * want
+
+
+# ---- DTcalculation (functions.cu:1684-1698) -------------------------------------------
+
+def test_dtcalculation_branches(oracle):
+    """courant = COURANT DL / max(c01, c02); DT > courant -> courant; else
+    courant < DT_MAX -> courant; else DT_MAX. With dt_mode = 1 every step runs it
+    first (main.cu:1030) and the simulated time advances by the new DT."""
+    case, p = _fluid_lattice(2, half=3)
+    p.courant = 0.5
+    cases = [  # (dt, dt_max, c01, c02, expected)
+        (1e-2, 1.0, 30.0, 20.0, 0.5 * p.dl / 30.0),        # DT > courant
+        (1e-6, 1.0, 20.0, 40.0, 0.5 * p.dl / 40.0),        # DT <= courant < DT_MAX
+        (1e-6, 1e-5, 20.0, 40.0, 1e-5),                    # courant >= DT_MAX
+        (1e-3, 1e-3, 25.0, 25.0, 0.5 * p.dl / 25.0),       # DT == DT_MAX > courant
+    ]
+    with vx.Simulation(p, case, backend=oracle) as s:
+        for dt, dtmax, c01, c02, want in cases:
+            q = s.get_params()
+            q.dt, q.dt_max, q.c01, q.c02 = dt, dtmax, c01, c02
+            s.set_params(q)
+            s.DTcalculation()
+            assert s.get_params().dt == want
+    dam = vx.dam_break_2d_c0()
+    q = vx.dam2d_params(dam)
+    q.courant, q.dt, q.dt_max, q.c01, q.c02, q.dt_mode = 0.1, 1e-2, 1.0, 30.0, 20.0, 1
+    courant = 0.1 * q.dl / 30.0
+    with vx.Simulation(q, dam, backend=oracle) as s:
+        s.init()
+        s.step(3)
+        st = s.status()
+        assert s.get_params().dt == courant
+    assert abs(st.time - 3 * courant) <= 1e-15
+
+
+# ---- COLLISION2 with unequal masses (functions.cu:1523-1601) ---------------------------
+
+@pytest.mark.parametrize("ti,tj", [(1, 2), (2, 1), (0, 1), (-1, 2), (0, 2)])
+def test_collision_unequal_masses(oracle, ti, tj):
+    """Head-on pair along x, i (lower id) approaching j: masses rho1 for PTYPE 1
+    and rho2 otherwise (walls and dummies too), impulse J = (1+cc) mu (u_i - u_j),
+    mu = m_i m_j/(m_i + m_j); fluid ends move by -J/m_i, +J/m_j (velocity) and
+    dt times that (position). Between two fluids momentum is conserved and the
+    relative velocity becomes -cc times its value, so j's row finds them
+    separating. A wall (or dummy) never moves: each visit multiplies the
+    fluid's velocity relative to it by k = 1 - (1+cc) m_w/(m_w + m_f), and with
+    k > 0 the fluid's own row hits the wall a second time: k^2 in all."""
+    l0 = 0.01
+    x = np.array([0.10, 0.10 + 0.4 * l0])
+    y = np.array([0.20, 0.20])
+    ui, uj = (0.7 if ti > 0 else 0.0), -0.4
+    case = vx.Case(2, l0, x, y, np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2), np.zeros(2),
+                   np.array([ti, tj], np.int32))
+    p = vx.default_params(2, l0)
+    p.rho2 = 400.0
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 0.3, 0.0, 0.3
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        s.set("UNEW", np.array([ui, uj]))
+        s.set("VNEW", np.zeros(2))
+        s.COLLISION2()
+        u, xx = s.get("UNEW"), s.get("X")
+        hits = s.status().collisions
+    mi = p.rho1 if ti == 1 else p.rho2
+    mj = p.rho1 if tj == 1 else p.rho2
+    if ti > 0:
+        J = (1 + p.cc) * mi * mj / (mi + mj) * (ui - uj)
+        assert hits == 1
+        assert np.allclose(u, [ui - J / mi, uj + J / mj], rtol=1e-14, atol=0)
+        assert np.allclose(xx, [x[0] - p.dt * J / mi, x[1] + p.dt * J / mj], rtol=1e-14, atol=0)
+        assert abs(mi * u[0] + mj * u[1] - (mi * ui + mj * uj)) <= 1e-12 * mi
+        assert abs((u[0] - u[1]) + p.cc * (ui - uj)) <= 1e-13
+    else:
+        k = 1.0 - (1 + p.cc) * mi / (mi + mj)
+        assert 0 < k < 1 and hits == 2
+        uf = ui + (uj - ui) * k * k
+        assert u[0] == ui and xx[0] == x[0]
+        assert abs(u[1] - uf) <= 1e-14
+        assert abs(xx[1] - (x[1] + p.dt * (uf - uj))) <= 1e-15
+
+
+def test_collision_oblique_3d(oracle):
+    """3D pair along n = (1, 2, 2)/3 with unequal masses and an arbitrary
+    relative velocity: the impulse acts along n only, J = (1+cc) mu ((u_i -
+    u_j) . n), so the tangential velocities are unchanged and the normal
+    relative velocity is reversed times cc."""
+    l0 = 0.01
+    d = 0.3 * l0
+    n = np.array([1.0, 2.0, 2.0]) / 3.0
+    P = np.array([[0.1, 0.1, 0.1], [0.1, 0.1, 0.1]])
+    P[1] += d * n
+    Vi, Vj = np.array([0.5, 0.2, -0.1]), np.array([-0.3, -0.4, 0.25])
+    case = vx.Case(3, l0, P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy(), np.zeros(2), np.zeros(2),
+                   np.zeros(2), np.zeros(2), np.array([1, 2], np.int32))
+    p = vx.default_params(3, l0)
+    p.rho2 = 600.0
+    p.xmin = p.ymin = p.zmin = 0.0
+    p.xmax = p.ymax = p.zmax = 0.3
+    with vx.Simulation(p, case, backend=oracle) as s:
+        s.init()
+        for k, f in enumerate(("UNEW", "VNEW", "WNEW")):
+            s.set(f, np.array([Vi[k], Vj[k]]))
+        s.COLLISION2()
+        V = np.stack([s.get(f) for f in ("UNEW", "VNEW", "WNEW")], 1)
+    mi, mj = p.rho1, p.rho2
+    dx = np.array([case.x[1] - case.x[0], case.y[1] - case.y[0], case.z[1] - case.z[0]])
+    nn = dx / np.linalg.norm(dx)
+    J = (1 + p.cc) * mi * mj / (mi + mj) * np.dot(Vi - Vj, nn)
+    assert J > 0
+    assert np.allclose(V[0], Vi - J * nn / mi, rtol=0, atol=1e-14)
+    assert np.allclose(V[1], Vj + J * nn / mj, rtol=0, atol=1e-14)
+    rel_t = (V[0] - V[1]) - np.dot(V[0] - V[1], nn) * nn
+    rel_t0 = (Vi - Vj) - np.dot(Vi - Vj, nn) * nn
+    assert np.allclose(rel_t, rel_t0, atol=1e-14)
+    assert abs(np.dot(V[0] - V[1], nn) + p.cc * np.dot(Vi - Vj, nn)) <= 1e-14

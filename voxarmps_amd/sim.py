@@ -124,6 +124,12 @@ class Simulation:
         self.be.check(self.be.fn("set_params")(self.ctx, C.byref(params)), "set_params")
         self.params = params.copy()
 
+    def get_params(self):
+        """The parameters in effect (DTcalculation updates dt in place)."""
+        q = self.params.copy()
+        self.be.check(self.be.fn("get_params")(self.ctx, C.byref(q)), "get_params")
+        return q
+
     # ---- stages, named after the reference routines ---------------------------------
     def _stage(self, name, *args):
         self.be.check(self.be.fn(name)(self.ctx, *args), name)
