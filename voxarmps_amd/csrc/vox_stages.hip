@@ -1170,9 +1170,15 @@ int st_collision(vox_ctx* c)
     // The margin widens on a violation and relaxes back towards its base over
     // the following steps.
     c->coll_margin = std::max(kCollMarginBase, 0.75 * c->coll_margin);
+    // Once coll + margin exceeds the list's own accept radius the screen takes
+    // every listed pair (radius +inf) and the motion check is off: the replay
+    // then evaluates exactly the pairs the reference loop visits, for any
+    // motion, even though the list was built at the step's earlier positions.
+    const double list_r = (p.dim == 3 || p.ref_gpu) ? p.re_s + p.delta : p.re_s;
     for (int attempt = 0; attempt < 8; ++attempt) {
         double margin = p.coll * c->coll_margin;
-        double rc = p.coll + margin;
+        const bool every_pair = p.coll + margin > list_r;
+        const double rc = every_pair ? HUGE_VAL : p.coll + margin;
         k_pack_pos<<<grid_for(n), kBlock, 0, c->stream>>>(n, p.dim, X, Y, Z, c->pos_rec);
         k_coll_screen<<<grid_for(n), kBlock, 0, c->stream>>>(n, rc, L, c->pos_rec, nullptr,
                                                               c->coll_cnt, nullptr, nullptr, 0);
@@ -1189,10 +1195,7 @@ int st_collision(vox_ctx* c)
                                                               c->coll_cap);
         CollArgs a;
         a.dim = p.dim; a.coll = p.coll; a.cc = p.cc; a.rho1 = p.rho1; a.rho2 = p.rho2; a.dt = p.dt;
-        // Once the screen radius covers the list's own accept radius every
-        // listed pair is a candidate and the replay is exact for any motion.
-        const double list_r = (p.dim == 3 || p.ref_gpu) ? p.re_s + p.delta : p.re_s;
-        a.half_margin = (rc > list_r) ? HUGE_VAL : 0.45 * margin;
+        a.half_margin = every_pair ? HUGE_VAL : 0.45 * margin;
         a.total = c->coll_off + n; a.ci = c->coll_i; a.cj = c->coll_j; a.pt = c->ptype;
         a.cap = c->coll_cap;
         a.x = X; a.y = Y; a.z = Z; a.u = U; a.v = V; a.w = W;
