@@ -283,6 +283,26 @@ int vox_get_cells(vox_ctx* ctx, int which, int32_t* cell);
 int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
                    int32_t* cols, double* vals, double* rhs, int32_t* ids);
 
+/* ---- PPE solve on the reference's device slot layout --------------------- */
+/* Replaces CGM (functions.cuh:60, functions.cu:506-716) and the CUDA path's
+ * INCDECOM + cgm1..cgm12 loop over d_poiss / d_neighb (main.cu:1159-1248,
+ * functions.cuh:176-198) for a caller that keeps those arrays. All pointers
+ * are device memory in the reference's 1-based layout: neighb and poiss hold
+ * (tp+1)*stride elements (stride = NEIGHBORS), neighb[i*stride+1] = count+1,
+ * neighb[i*stride+l] (l = 2..count+1) a 1-based neighbour id, poiss[i*stride+1]
+ * the diagonal and poiss[i*stride+l] that neighbour's coefficient
+ * (functions.cu:1007-1060); source, bcon and pnew hold tp+1 elements. Rows
+ * with bcon == 0 are the unknowns and a neighbour with bcon != 0 is a
+ * Dirichlet p = 0. Jacobi-PCG from pnew = 0 to ||r|| <= tol ||b|| (FP64, dot
+ * products in a fixed order); pnew receives the solution and 0 on the other
+ * rows. Runs on the null stream of the current device and returns when done.
+ * The CG workspace (3 (tp+1) doubles) is allocated and freed inside the call,
+ * as the reference's CGM does (functions.cu:508-512). info->unknowns and
+ * info->nnz are -1 (not counted). */
+int vox_cgm_slot(int tp, int stride, const int32_t* neighb, const double* poiss,
+                 const double* source, const int32_t* bcon, double* pnew, double tol, int maxit,
+                 vox_solve_info* info);
+
 /* ---- learned preconditioner (no counterpart in the reference) ------------ */
 /* A message-passing network over the PPE matrix graph (hidden width 64,
  * messages of 16, 3 layers, bf16 weights on the matrix cores with FP32

@@ -81,3 +81,100 @@ def random_state(sim, seed, with_pressure=True):
         sim.set("PHAT", rng.uniform(0, 1000, n))
         sim.set("P", rng.uniform(0, 3000, n))
     return rng
+
+
+class HipMem:
+    """Device buffers through the HIP runtime that libvoxarmps itself loaded
+    (found in this process's mappings, so both share one runtime instance)."""
+
+    def __init__(self, backend):
+        import ctypes as C
+        path = None
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64.so" in line:
+                    path = line.split()[-1]
+                    break
+        if path is None:
+            raise RuntimeError("the HIP runtime is not loaded")
+        self.C = C
+        self.rt = C.CDLL(path)
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.rt.hipDeviceSynchronize.argtypes = []
+        self.live = []
+
+    def put(self, arr):
+        """Copy a contiguous numpy array to a new device buffer; returns its address."""
+        C = self.C
+        a = np.ascontiguousarray(arr)
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), max(a.nbytes, 8)) == 0
+        assert self.rt.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0      # host to device
+        self.live.append(p.value)
+        return p.value
+
+    def get(self, ptr, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert self.rt.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.live:
+            self.rt.hipFree(p)
+        self.live = []
+
+
+def slot_system(sim, rng=None):
+    """The context's assembled PPE (after MATRIX) in the reference's 1-based
+    slot layout of the CUDA path (functions.cu:1007-1060): neighb and poiss of
+    (tp+1)*stride elements over list L, the diagonal in slot 1, the source and
+    bcon of tp+1 elements. Neighbours with bcon == 1 get a coefficient as the
+    reference writes one (the solve must treat them as p = 0); dummy
+    neighbours and the particle itself get 0. Returns (stride, neighb, poiss,
+    source, bcon, csr) with csr the exported system."""
+    import scipy.sparse as sp
+    tp = sim.tp
+    cnt, ids = sim.neighbors(vx.LIST_L)
+    stride = int(cnt.max()) + 2
+    neighb = np.zeros((tp + 1) * stride, np.int32)
+    poiss = np.zeros((tp + 1) * stride, np.float64)
+    source = np.zeros(tp + 1, np.float64)
+    bcon = np.zeros(tp + 1, np.int32)
+    b = sim.get("BCON")
+    bcon[1:] = b
+    csr = sim.ppe()
+    nu = len(csr["rhs"])
+    A = sp.csr_matrix((csr["vals"], csr["cols"], csr["rowptr"]), shape=(nu, nu))
+    uid = np.full(tp, -1, np.int64)
+    uid[csr["ids"]] = np.arange(nu)
+    rng = rng or np.random.default_rng(0)
+    coo = A.tocoo()
+    keys = coo.row.astype(np.int64) * nu + coo.col
+    order = np.argsort(keys)
+    keys, kval = keys[order], coo.data[order]
+    rows = np.nonzero(uid >= 0)[0]
+    base = (np.arange(tp) + 1) * stride
+    neighb[base + 1] = cnt + 1
+    lane = np.arange(ids.shape[1])[None, :]
+    valid = lane < cnt[:, None]
+    I = np.broadcast_to(np.arange(tp)[:, None], ids.shape)[valid]
+    L = np.broadcast_to(lane, ids.shape)[valid]
+    J = ids[valid].astype(np.int64)
+    neighb[base[I] + 2 + L] = J + 1
+    r = uid[rows]
+    poiss[base[rows] + 1] = A.diagonal()[r]
+    source[rows + 1] = csr["rhs"][r]
+    m = (uid[I] >= 0) & (J != I) & (b[J] != -1)
+    I, L, J = I[m], L[m], J[m]
+    unk = uid[J] >= 0
+    k = uid[I[unk]] * nu + uid[J[unk]]
+    pos = np.clip(np.searchsorted(keys, k), 0, len(keys) - 1)
+    hit = keys[pos] == k
+    v = np.zeros(len(k))
+    v[hit] = kval[pos[hit]]
+    poiss[base[I[unk]] + 2 + L[unk]] = v
+    ns = int(np.count_nonzero(~unk))
+    poiss[base[I[~unk]] + 2 + L[~unk]] = -rng.uniform(1e3, 1e5, ns)
+    return stride, neighb, poiss, source, bcon, {"A": A, "rhs": csr["rhs"], "ids": csr["ids"]}

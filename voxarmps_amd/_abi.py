@@ -158,6 +158,8 @@ PROTOTYPES = {
     "vox_lpc_loss_grad": (C.c_int, [_P, C.c_int32, C.c_uint64, _dp, _fp]),
     "vox_lpc_train": (C.c_int, [_P, C.c_int32, C.c_double, C.c_uint64, _dp, _dp]),
     "vox_attach_lpc": (C.c_int, [_P, _P]),
+    "vox_cgm_slot": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_double, C.c_int,
+                               C.POINTER(VoxSolveInfo)]),
 }
 
 

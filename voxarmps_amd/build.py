@@ -15,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvoxarmps.so")
 SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_stages32.hip", "vox_ppe.hip",
+           "vox_cgm.hip",
            "vox_cg.hip", "vox_lpc.hip", "vox_io.cpp"]
 # The FP32 stages of the mixed-precision mode have no bit-exact counterpart, so
 # they let the compiler fuse multiply-adds (the last -ffp-contract wins).
