@@ -302,6 +302,15 @@ int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
 int vox_cgm_slot(int tp, int stride, const int32_t* neighb, const double* poiss,
                  const double* source, const int32_t* bcon, double* pnew, double tol, int maxit,
                  vox_solve_info* info);
+/* No counterpart in the reference: vox_cgm_slot for callers that cannot
+ * allocate device memory (code under stream capture, serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, kernels and results (bit for bit);
+ * the workspace is a static array of the library, one per GPU (24.1 MiB),
+ * so tp is limited to 1,048,575 (VOX_ERR_ARG beyond). It calls no hipMalloc
+ * or hipFree. Calls that overlap on one GPU share that array and race. */
+int vox_cgm_slot_noalloc(int tp, int stride, const int32_t* neighb, const double* poiss,
+                         const double* source, const int32_t* bcon, double* pnew, double tol,
+                         int maxit, vox_solve_info* info);
 
 /* ---- learned preconditioner (no counterpart in the reference) ------------ */
 /* A message-passing network over the PPE matrix graph (hidden width 64,

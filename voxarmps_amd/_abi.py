@@ -160,6 +160,8 @@ PROTOTYPES = {
     "vox_attach_lpc": (C.c_int, [_P, _P]),
     "vox_cgm_slot": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_double, C.c_int,
                                C.POINTER(VoxSolveInfo)]),
+    "vox_cgm_slot_noalloc": (C.c_int, [C.c_int, C.c_int, _P, _P, _P, _P, _P, C.c_double, C.c_int,
+                                       C.POINTER(VoxSolveInfo)]),
 }
 
 

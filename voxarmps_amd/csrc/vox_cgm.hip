@@ -21,6 +21,8 @@
 //   dir      convergence test on the device; p = r/d + beta p
 // Kernels launched after the device has set its stop flag return at entry,
 // so the host reads the flags once per batch of iterations.
+#include <mutex>
+
 #include "vox_internal.hpp"
 
 namespace vox {
@@ -288,9 +290,55 @@ static int slot_run(int tp, int stride, const int32_t* neighb, const double* poi
     return VOX_OK;
 }
 
+// ---- allocation-free variant ---------------------------------------------------------
+// For callers that cannot allocate device memory (code under stream capture,
+// serving loops): the workspace is this file-scope array, one copy per GPU,
+// sized for tp <= kNoAllocMaxTp (24.1 MiB). Its device address is looked up
+// once per device with that device current; host code uses that address,
+// never the array's own name. Every call on a GPU shares the array, so calls
+// that overlap on one GPU race; each call initialises everything it reads.
+constexpr long kNoAllocMaxTp = (1L << 20) - 1;
+static __device__ __align__(256) double scratch[slot_ws_doubles(kNoAllocMaxTp)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_for_current_device(double** out)
+{
+    int dev = 0;
+    VOX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(VOX_ERR_ARG, "vox_cgm_slot_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        VOX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(VOX_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return VOX_OK;
+}
+
 }  // namespace vox
 
 extern "C" {
+
+int vox_cgm_slot_noalloc(int tp, int stride, const int32_t* neighb, const double* poiss,
+                         const double* source, const int32_t* bcon, double* pnew, double tol,
+                         int maxit, vox_solve_info* info)
+{
+    using namespace vox;
+    VOX_TRY(slot_check_args(tp, stride, neighb, poiss, source, bcon, pnew, tol, maxit));
+    if (tp > kNoAllocMaxTp)
+        return fail(VOX_ERR_ARG, "vox_cgm_slot_noalloc: tp = " + std::to_string(tp) +
+                                     " exceeds the static workspace limit of " +
+                                     std::to_string(kNoAllocMaxTp) + " particles");
+    double* ws = nullptr;
+    VOX_TRY(scratch_for_current_device(&ws));
+    return slot_run(tp, stride, neighb, poiss, source, bcon, pnew, tol, maxit, info, ws);
+}
 
 int vox_cgm_slot(int tp, int stride, const int32_t* neighb, const double* poiss,
                  const double* source, const int32_t* bcon, double* pnew, double tol, int maxit,
