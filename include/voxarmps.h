@@ -181,6 +181,24 @@ int vox_params_default(vox_params* p, int dim, double dl);
 /* Grid box from positions: min/max over particles expanded by DL (main.cu:692-715). */
 int vox_params_box(vox_params* p, int tp, const double* x, const double* y, const double* z);
 
+/* The run main.cu sets up from its 16 command-line arguments
+ * (main.cu:65-209): test ("dam", "drop", "shear", "turb", "oil"), the
+ * viscoplastic / turbulence / compressibility / matopt / srcopt switches,
+ * code_opt ("sequential" or "openmp": the reference's CPU path; "gpu": its
+ * CUDA path, ref_gpu = 1), 3D (the dam test only), the two fluids and the
+ * time step (at most 0.005 s). Sets the constants those select
+ * (main.cu:211-494: spacing, radius, kernel, lambda, relaxation, collision,
+ * gravity, rheology, method, DTcalculation every step), the box of the tp
+ * input positions (main.cu:692-715, widened for the drop test) and PMAX =
+ * 4.5 tp (main.cu:548). The drop test's velocity override u = -x, v = y
+ * (main.cu:645-651) is the caller's, on the input arrays. */
+int vox_params_main(vox_params* p, const char* test, int visc, int turb, int comp, int matopt,
+                    int srcopt, const char* code_opt, int threedim, double rho1, double nu1,
+                    double rho2, double nu2, double dt, int tp, const double* x, const double* y,
+                    const double* z);
+/* Input file name main.cu opens for a test (main.cu:505-533). */
+const char* vox_main_input(const char* test, int threedim);
+
 /* Allocate device state for tp particles ordered ghost -> wall -> fluid
  * (gp, wp, fp counts; main.cu:635-682). */
 int vox_create(vox_ctx** ctx, const vox_params* p, int tp, int gp, int wp, int fp);
@@ -282,6 +300,13 @@ int vox_get_cells(vox_ctx* ctx, int which, int32_t* cell);
  * (particle id of each unknown). */
 int vox_ppe_export(vox_ctx* ctx, int32_t* nu, int32_t* nnz, int32_t* rowptr,
                    int32_t* cols, double* vals, double* rhs, int32_t* ids);
+/* The inverse of vox_ppe_export (the "load" half of the PPE dump
+ * for training data and checks; no counterpart in the reference): a CSR
+ * system in the export's form (rowptr[nu+1], each row's diagonal first, cols
+ * as unknown indices, ids the distinct particle of each unknown) replaces the
+ * context's assembled PPE, so that vox_ppe_solve solves it. Requires vox_init. */
+int vox_ppe_load(vox_ctx* ctx, int32_t nu, int32_t nnz, const int32_t* rowptr,
+                 const int32_t* cols, const double* vals, const double* rhs, const int32_t* ids);
 
 /* ---- PPE solve on the reference's device slot layout --------------------- */
 /* Replaces CGM (functions.cuh:60, functions.cu:506-716) and the CUDA path's
@@ -382,8 +407,18 @@ int vox_grd_write(const char* path, int dim, int tp, const double* x, const doub
                   const double* z, const double* u, const double* v, const double* w,
                   const double* p, const int32_t* ptype);
 /* Write an ASCII UnstructuredGrid VTU of all particles (3D writes z and w,
- * which the reference omits: inOut.cpp:132-136). fluid_only writes PTYPE > 0. */
+ * which the reference omits: inOut.cpp:132-136). fluid_only writes PTYPE > 0.
+ * Values are Float64 with 17 significant digits (restart-grade), in the line
+ * layout the reference's getVTU reads (inOut.cpp:43-102). */
 int vox_write_vtu(vox_ctx* ctx, const char* path, int fluid_only);
+/* Point count of a VTU (getTP, inOut.cpp:25-41). */
+int vox_vtu_count(const char* path, int* tp);
+/* Restart state from a VTU (getVTU, inOut.cpp:43-102): positions,
+ * velocities, pressure and type of tp points into caller arrays (z, w
+ * ignored in 2D) and the ghost / wall / fluid counts. */
+int vox_vtu_read(const char* path, int dim, int tp, double* x, double* y, double* z,
+                 double* u, double* v, double* w, double* p, int32_t* ptype,
+                 int* gp, int* wp, int* fp);
 
 #ifdef __cplusplus
 }

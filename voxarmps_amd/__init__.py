@@ -8,12 +8,12 @@ from .cases import (Case, batch_cases, box_of, dam_break, dam_break_2d_c0, dam_b
                     ensemble_2d, ensemble_c4, jitter, jittered_dam_breaks_2d, two_phase)
 from .params import (dam2d_params, dam3d_params, default_params, ensemble_params,  # noqa: F401
                      set_box, two_radius, wc_params)
-from .sim import Backend, Simulation, VoxError, hip_backend  # noqa: F401
+from .sim import Backend, Simulation, VoxError, hip_backend, read_vtu  # noqa: F401
 from .lpc import LearnedPreconditioner  # noqa: F401
 
 __all__ = [
     "F", "FIELDS", "LIST_L", "LIST_S", "VoxParams", "VoxSolveInfo", "VoxStatus", "Case",
     "dam_break", "dam_break_2d_c0", "dam_break_3d_c1", "jitter", "dam2d_params", "dam3d_params",
     "default_params", "set_box", "two_radius", "Backend", "Simulation", "VoxError", "hip_backend",
-    "LearnedPreconditioner",
+    "LearnedPreconditioner", "read_vtu",
 ]

@@ -101,6 +101,10 @@ PROTOTYPES = {
     "vox_device_count": (C.c_int, [_ip]),
     "vox_params_default": (C.c_int, [_pp, C.c_int, C.c_double]),
     "vox_params_box": (C.c_int, [_pp, C.c_int, _dp, _dp, _dp]),
+    "vox_params_main": (C.c_int, [_pp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]),
+    "vox_main_input": (C.c_char_p, [C.c_char_p, C.c_int]),
     "vox_create": (C.c_int, [C.POINTER(_P), _pp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "vox_destroy": (C.c_int, [_P]),
     "vox_get_params": (C.c_int, [_P, _pp]),
@@ -138,6 +142,10 @@ PROTOTYPES = {
     "vox_get_neighbors_slot": (C.c_int, [_P, C.c_int, _i32p, C.c_int]),
     "vox_get_cells": (C.c_int, [_P, C.c_int, _i32p]),
     "vox_ppe_export": (C.c_int, [_P, _i32p, _i32p, _i32p, _i32p, _dp, _dp, _i32p]),
+    "vox_ppe_load": (C.c_int, [_P, C.c_int32, C.c_int32, _i32p, _i32p, _dp, _dp, _i32p]),
+    "vox_vtu_count": (C.c_int, [C.c_char_p, _ip]),
+    "vox_vtu_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                               _i32p, _ip, _ip, _ip]),
     "vox_prof_enable": (C.c_int, [_P, C.c_int]),
     "vox_prof_only": (C.c_int, [_P, C.c_char_p]),
     "vox_prof_get": (C.c_int, [_P, C.c_char_p, _dp, _i64p]),

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvoxarmps.so")
+CLI = os.path.join(HERE, "voxmps")
 SOURCES = ["vox_api.hip", "vox_neighbor.hip", "vox_stages.hip", "vox_stages32.hip", "vox_ppe.hip",
            "vox_cgm.hip",
            "vox_cg.hip", "vox_lpc.hip", "vox_io.cpp"]
@@ -57,7 +58,24 @@ def build_library(verbose=False, defines=(), out=LIB):
     os.replace(tmp, LIBOUT)
     if verbose:
         print(f"built {LIBOUT}", file=sys.stderr)
+    if out == LIB:
+        build_cli(verbose)
     return LIBOUT
+
+
+def build_cli(verbose=False):
+    """voxmps, the reference's 16-argument command line over the library
+    (host C++ only, linked against libvoxarmps.so next to it)."""
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(CSRC, "vox_cli.cpp"), "-o", tmp,
+           "-L" + HERE, "-lvoxarmps", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"voxmps build failed:\n{r.stderr}")
+    os.replace(tmp, CLI)
+    if verbose:
+        print(f"built {CLI}", file=sys.stderr)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -241,6 +241,75 @@ int vox_params_box(vox_params* p, int tp, const double* x, const double* y, cons
     return VOX_OK;
 }
 
+// The run main.cu sets up from its 16 command-line arguments: the test and
+// option words (main.cu:88-209), the constants they select (211-494), the
+// input's box (692-715) and PMAX = 4.5 TP (548).
+int vox_params_main(vox_params* p, const char* test, int visc, int turb, int comp, int matopt,
+                    int srcopt, const char* code_opt, int threedim, double rho1, double nu1,
+                    double rho2, double nu2, double dt, int tp, const double* x, const double* y,
+                    const double* z)
+{
+    if (!p || !test || !code_opt) return fail(VOX_ERR_ARG, "null argument");
+    std::string t(test), code(code_opt);
+    if (t != "oil" && t != "drop" && t != "shear" && t != "turb") t = "dam";      // main.cu:88-108
+    if (code != "openmp" && code != "gpu" && code != "sequential")                // main.cu:154-171
+        return fail(VOX_ERR_ARG, "invalid code optimization option (openmp, gpu or sequential)");
+    if (!(dt > 0.0) || dt > 0.005)                                              // main.cu:396-402
+        return fail(VOX_ERR_ARG, "time-step duration must be in (0, 0.005] s");
+    const bool compressible = comp || t == "oil";                               // main.cu:128-138
+    const bool three = threedim && t == "dam";                                  // main.cu:173-183
+    if (three && code == "gpu")
+        return fail(VOX_ERR_ARG, "the reference's CUDA path is 2D only");
+    const double dl = compressible ? 0.0040 : 0.010;                            // main.cu:248-250
+    VOX_TRY(vox_params_default(p, three ? 3 : 2, dl));
+    const double re = p->re_s;                                                  // 3.2 DL / 4.0 DL
+    p->relaxp = compressible ? 0.5 : 0.2;                                       // main.cu:263-265
+    p->cc = compressible ? 0.50 : 0.20;                                         // main.cu:277-279
+    p->ktype = compressible ? 6 : 2;                                            // main.cu:288-292
+    p->method = compressible ? 3 : 1;                                           // main.cu:335-339
+    p->turb = (t == "turb" || turb) ? 1 : 0;                                    // main.cu:303-312
+    p->gy = (t == "turb" || t == "shear") ? 0.0 : -9.8;                          // main.cu:316-328
+    p->fluid2_type = visc ? 1 : 0;                                              // main.cu:362-377
+    if (visc) { p->n_pl = 1.3; p->meu0 = 0.04; p->phi = 0.0; p->cohes = 0.0; }
+    else { p->n_pl = 0.0; p->meu0 = 0.0; p->phi = 0.0; p->cohes = 0.0; }
+    p->rho1 = rho1; p->nu1 = nu1; p->rho2 = rho2; p->nu2 = nu2;                 // main.cu:244-247
+    p->lambda = (p->ktype == 6 ? 0.14288 : 0.16832) * (re * re);                // main.cu:490-495
+    p->dt = dt;
+    p->dt_mode = 1;                                                             // main.cu:1030
+    p->vel_cor = (t == "turb") ? 0.8 : 1.0;                                     // main.cu:509-512
+    p->test_kind = (t == "drop") ? VOX_TEST_DROP : (t == "turb") ? VOX_TEST_TURB : VOX_TEST_DAM;
+    p->matopt = matopt ? 1 : 0;
+    p->srcopt = srcopt ? 1 : 0;
+    p->ref_gpu = (code == "gpu") ? 1 : 0;
+    if (tp > 0 && x && y) {
+        VOX_TRY(vox_params_box(p, tp, x, y, z));                                // main.cu:704-715
+        if (t == "drop") {                                                      // main.cu:692-703
+            double xmn = x[0], xmx = x[0], ymn = y[0], ymx = y[0];
+            for (int i = 1; i < tp; ++i) {
+                xmn = std::min(xmn, x[i]); xmx = std::max(xmx, x[i]);
+                ymn = std::min(ymn, y[i]); ymx = std::max(ymx, y[i]);
+            }
+            p->xmin = xmn - 100 * p->dl;
+            p->ymin = ymn - 10000 * p->dl;
+            p->xmax = xmx + 100 * p->dl;
+            p->ymax = ymx + 100 * p->dl;
+        }
+        p->pmax = 4.50 * (double)tp;                                            // main.cu:548
+    }
+    return VOX_OK;
+}
+
+const char* vox_main_input(const char* test, int threedim)
+{
+    const std::string t = test ? test : "";
+    if (threedim && t != "oil" && t != "drop" && t != "shear" && t != "turb") return "input-dam3d.grd";
+    if (t == "turb") return "input-turb.grd";                                   // main.cu:505-533
+    if (t == "shear") return "input-shear.grd";
+    if (t == "drop") return "input-drop.grd";
+    if (t == "oil") return "input-oil.grd";
+    return "input-dam.grd";
+}
+
 static int validate(const vox_params* p)
 {
     if (p->dim != 2 && p->dim != 3) return fail(VOX_ERR_ARG, "dim must be 2 or 3");
@@ -693,6 +762,13 @@ int vox_ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32
     VOX_TRY(need_init(c));
     VOX_HIP(hipStreamSynchronize(c->stream));
     return ppe_export(c, nu, nnz, rowptr, cols, vals, rhs, ids);
+}
+
+int vox_ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const int32_t* cols,
+                 const double* vals, const double* rhs, const int32_t* ids)
+{
+    VOX_TRY(need_init(c));
+    return ppe_load(c, nu, nnz, rowptr, cols, vals, rhs, ids);
 }
 
 int vox_prof_enable(vox_ctx* c, int on)

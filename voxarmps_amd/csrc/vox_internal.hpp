@@ -384,6 +384,8 @@ int st_pressure_wc(vox_ctx* c);   // PRESSURECALC method 3
 int ppe_solve(vox_ctx* c, vox_solve_info* info);
 int ppe_export(vox_ctx* c, int32_t* nu, int32_t* nnz, int32_t* rowptr, int32_t* cols,
                double* vals, double* rhs, int32_t* ids);
+int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const int32_t* cols,
+             const double* vals, const double* rhs, const int32_t* ids);
 
 // learned preconditioner (vox_lpc.hip): features, network and coupling
 // operator for the context's assembled PPE

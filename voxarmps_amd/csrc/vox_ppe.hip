@@ -10,6 +10,8 @@
 // rows is one contiguous block, which is what the product reads; only nonzero
 // unknown columns are stored. The solver lives in vox_cg.hip.
 //
+#include <algorithm>
+
 #include "vox_internal.hpp"
 
 namespace vox {
@@ -468,6 +470,75 @@ int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, i
         if (ids) ids[r] = urow[r];
     }
     if (rowptr) rowptr[nu] = (int32_t)k;
+    return VOX_OK;
+}
+
+// Load a PPE given as CSR over nu unknowns (the diagonal first in each row,
+// as ppe_export writes it) in place of the assembled one: the inverse of
+// ppe_export. The rows go to the same SELL-64 storage the assembly fills.
+int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const int32_t* cols,
+             const double* vals, const double* rhs, const int32_t* ids)
+{
+    if (nu < 0 || nu > c->tp) return fail(VOX_ERR_ARG, "vox_ppe_load: nu outside 0..tp");
+    if (nu > 0 && (!rowptr || !cols || !vals || !rhs || !ids))
+        return fail(VOX_ERR_ARG, "vox_ppe_load: null array");
+    if (nu > 0 && (rowptr[0] != 0 || rowptr[nu] != nnz))
+        return fail(VOX_ERR_ARG, "vox_ppe_load: rowptr[0] must be 0 and rowptr[nu] == nnz");
+    int maxoff = 0;
+    std::vector<char> seen(c->tp, 0);
+    for (int r = 0; r < nu; ++r) {
+        const int a = rowptr[r], b = rowptr[r + 1];
+        if (b <= a) return fail(VOX_ERR_ARG, "vox_ppe_load: row " + std::to_string(r) + " is empty");
+        if (cols[a] != r)
+            return fail(VOX_ERR_ARG, "vox_ppe_load: row " + std::to_string(r) +
+                                         " does not start with its diagonal");
+        for (int k = a + 1; k < b; ++k)
+            if (cols[k] < 0 || cols[k] >= nu)
+                return fail(VOX_ERR_ARG, "vox_ppe_load: column out of range in row " +
+                                             std::to_string(r));
+        maxoff = std::max(maxoff, b - a - 1);
+        if (ids[r] < 0 || ids[r] >= c->tp || seen[ids[r]])
+            return fail(VOX_ERR_ARG, "vox_ppe_load: particle ids must be distinct and in 0..tp-1");
+        seen[ids[r]] = 1;
+    }
+    const int cap = round_cap(std::max(maxoff, 1));
+    if (c->a_cap < cap) {
+        for (int32_t** q : {&c->acol, &c->tcol}) { if (*q) (void)hipFree(*q); *q = nullptr; }
+        for (double** q : {&c->aval, &c->tval}) { if (*q) (void)hipFree(*q); *q = nullptr; }
+        VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
+        VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(c->tp, cap)));
+        VOX_HIP(hipMalloc(&c->tcol, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
+        VOX_HIP(hipMalloc(&c->tval, sizeof(double) * (size_t)sl_size(c->tp, cap)));
+        if (c->aval32) (void)hipFree(c->aval32);
+        c->aval32 = nullptr;
+        c->a_cap = cap;
+    }
+    const long tot = sl_size(nu, c->a_cap);
+    std::vector<int32_t> len(nu > 0 ? nu : 1), acol(nu > 0 ? tot : 1, 0);
+    std::vector<double> aval(nu > 0 ? tot : 1, 0.0), diag(nu > 0 ? nu : 1);
+    std::vector<int32_t> uidx(c->tp, -1);
+    for (int r = 0; r < nu; ++r) {
+        const int a = rowptr[r], b = rowptr[r + 1];
+        diag[r] = vals[a];
+        len[r] = b - a - 1;
+        for (int k = a + 1; k < b; ++k) {
+            acol[mx_index(r, k - a - 1, c->a_cap)] = cols[k];
+            aval[mx_index(r, k - a - 1, c->a_cap)] = vals[k];
+        }
+        uidx[ids[r]] = r;
+    }
+    VOX_HIP(hipStreamSynchronize(c->stream));
+    if (nu > 0) {
+        VOX_HIP(hipMemcpy(c->rowlen, len.data(), sizeof(int32_t) * nu, hipMemcpyHostToDevice));
+        VOX_HIP(hipMemcpy(c->acol, acol.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice));
+        VOX_HIP(hipMemcpy(c->aval, aval.data(), sizeof(double) * tot, hipMemcpyHostToDevice));
+        VOX_HIP(hipMemcpy(c->adiag, diag.data(), sizeof(double) * nu, hipMemcpyHostToDevice));
+        VOX_HIP(hipMemcpy(c->rhs, rhs, sizeof(double) * nu, hipMemcpyHostToDevice));
+        VOX_HIP(hipMemcpy(c->urow, ids, sizeof(int32_t) * nu, hipMemcpyHostToDevice));
+    }
+    VOX_HIP(hipMemcpy(c->uidx, uidx.data(), sizeof(int32_t) * c->tp, hipMemcpyHostToDevice));
+    c->nu = nu;
+    c->aval32_ok = false;
     return VOX_OK;
 }
 

@@ -267,3 +267,43 @@ class Simulation:
                          _ptr(cols, C.c_int32), _ptr(vals, C.c_double), _ptr(rhs, C.c_double),
                          _ptr(ids, C.c_int32)), "ppe_export")
         return {"rowptr": rowptr, "cols": cols, "vals": vals, "rhs": rhs, "ids": ids}
+
+
+def _sim_write_vtu(self, path, fluid_only=False):
+    """vox_write_vtu: every particle (or the fluid only) at full precision."""
+    self.be.check(self.be.fn("write_vtu")(self.ctx, str(path).encode(), 1 if fluid_only else 0),
+                  "write_vtu")
+
+
+def _sim_load_ppe(self, system):
+    """vox_ppe_load: a CSR system in the form ppe() returns replaces the
+    assembled PPE (the next CGM solves it)."""
+    nu = int(system["rhs"].shape[0])
+    i32 = {k: np.ascontiguousarray(system[k], dtype=np.int32) for k in ("rowptr", "cols", "ids")}
+    f64 = {k: np.ascontiguousarray(system[k], dtype=np.float64) for k in ("vals", "rhs")}
+    self.be.check(self.be.fn("ppe_load")(
+        self.ctx, nu, int(i32["rowptr"][-1]) if nu else 0, _ptr(i32["rowptr"], C.c_int32),
+        _ptr(i32["cols"], C.c_int32), _ptr(f64["vals"], C.c_double), _ptr(f64["rhs"], C.c_double),
+        _ptr(i32["ids"], C.c_int32)), "ppe_load")
+
+
+Simulation.write_vtu = _sim_write_vtu
+Simulation.load_ppe = _sim_load_ppe
+
+
+def read_vtu(path, dim, l0, backend=None):
+    """A Case from a VTU restart file (vox_vtu_count / vox_vtu_read, the
+    reference's getTP / getVTU of inOut.cpp:25-102)."""
+    from .cases import Case
+    be = backend or hip_backend()
+    n = C.c_int(0)
+    be.check(be.lib.vox_vtu_count(str(path).encode(), C.byref(n)), "vtu_count")
+    tp = n.value
+    a = {k: np.zeros(tp) for k in ("x", "y", "z", "u", "v", "w", "p")}
+    t = np.zeros(tp, np.int32)
+    g, w, f = C.c_int(), C.c_int(), C.c_int()
+    be.check(be.lib.vox_vtu_read(str(path).encode(), dim, tp, *(_ptr(a[k], C.c_double) for k in
+                                                                ("x", "y", "z", "u", "v", "w", "p")),
+                                 _ptr(t, C.c_int32), C.byref(g), C.byref(w), C.byref(f)), "vtu_read")
+    return Case(dim, l0, a["x"], a["y"], a["z"], a["u"], a["v"], a["w"], a["p"], t,
+                name=f"vtu:{path}", meta={"counts": (g.value, w.value, f.value)})
