@@ -183,11 +183,16 @@ def stream_bandwidth(mib):
                       "sum_read_GBps": n * 8 / ts / 1e9}))
 
 
-def precision_report(checkpoints, scale=1):
+STATE_FIELDS = ("X", "Y", "Z", "U", "V", "W", "P", "PNEW")
+
+
+def precision_report(checkpoints, scale=1, relaxp=None):
     """The mixed-precision run against the all-FP64 run of the same workload,
     both on the GPU from the same initial state: relative L2 and Linf
     differences of the pressure field (and of the fluid positions) at each
-    checkpoint step. Prints one JSON line per checkpoint."""
+    checkpoint step, and the error of a single step: both runs restart from
+    the FP64 run's state at the checkpoint and take one step.
+    Prints one JSON line per checkpoint."""
     import numpy as np
 
     import voxarmps_amd as vx
@@ -195,6 +200,8 @@ def precision_report(checkpoints, scale=1):
     be = vx.hip_backend()
     case = vx.dam_break_3d_c1(scale=scale)
     p64 = vx.dam3d_params(case)
+    if relaxp is not None:
+        p64.relaxp = relaxp
     p32 = p64.copy()
     p32.precision = vx._abi.PREC_MIXED
     a = vx.Simulation(p64, case, backend=be)
@@ -219,16 +226,42 @@ def precision_report(checkpoints, scale=1):
         d = pb - pa
         f = case.ptype > 0
         dx = max(float(np.max(np.abs(b.get(k)[f] - a.get(k)[f]))) for k in ("X", "Y", "Z"))
-        print(json.dumps({
+        rec = {
             "step": cp, "particles": case.tp,
             "p_l2_rel": float(np.linalg.norm(d) / max(np.linalg.norm(pa), 1e-300)),
             "p_linf_rel": float(np.max(np.abs(d)) / max(np.max(np.abs(pa)), 1e-300)),
             "p_linf_abs_pa": float(np.max(np.abs(d))),
             "p_max_pa": float(np.max(np.abs(pa))),
+            "p_mean_fluid_pa": float(np.mean(pa[f])),
+            "front_m": float(a.get("X")[f].max()), "height_m": float(a.get("Y")[f].max()),
             "fluid_pos_linf_m": dx,
-            "iters_fp64": a.status().last_iters, "iters_mixed": b.status().last_iters,
+            "iters_fp64": a.status().last_iters,
+            "iters_mixed": b.status().last_iters,
             "relres_mixed": b.status().last_relres,
-        }), flush=True)
+        }
+        # The trajectories part after a few hundred steps (collisions, the CG
+        # tolerance), so the error of one step is measured as well: both runs
+        # restart from the FP64 run's state, with the neighbour lists (which
+        # PNUM reads before the step rebuilds them) built from it, and take
+        # one step each.
+        state = {k: a.get(k) for k in STATE_FIELDS}
+        for sim in (a, b):
+            for k in STATE_FIELDS:
+                sim.set(k, state[k])
+            sim.NEIGHBOR()
+        a.step(1)
+        b.step(1)
+        done += 1
+        for k in ("P", "U", "V", "W", "X", "Y", "Z"):
+            va, vb = a.get(k), b.get(k)
+            if k == "P":
+                rec["step1_p_l2_rel"] = float(np.linalg.norm(vb - va) / max(np.linalg.norm(va), 1e-300))
+                rec["step1_p_linf_rel"] = float(np.max(np.abs(vb - va)) / max(np.max(np.abs(va)), 1e-300))
+            else:
+                key = "step1_vel_linf_rel" if k in ("U", "V", "W") else "step1_pos_linf_m"
+                sc = max(float(np.max(np.abs(va[f]))), 1e-300) if k in ("U", "V", "W") else 1.0
+                rec[key] = max(rec.get(key, 0.0), float(np.max(np.abs(vb[f] - va[f]))) / sc)
+        print(json.dumps(rec), flush=True)
     a.close()
     b.close()
 
@@ -289,7 +322,7 @@ STAGES_TO_MATRIX = ("PNUM", "NEIGHBOR", "SPS", "V_FRACTION", "VISCOSITY", "PREDI
                     "BCON", "MATRIX")
 
 
-def ppe_report(checkpoints, scale=1, weights=None):
+def ppe_report(checkpoints, scale=1, weights=None, relaxp=None):
     """The FP64 run of the 3D dam break; at each checkpoint step the next
     step's PPE is assembled (the stages up to MATRIX, as the step runs them)
     and solved by every variant of ppe_compare. One JSON line per checkpoint.
@@ -301,6 +334,8 @@ def ppe_report(checkpoints, scale=1, weights=None):
     be = vx.hip_backend()
     case = vx.dam_break_3d_c1(scale=scale)
     p = vx.dam3d_params(case)
+    if relaxp is not None:
+        p.relaxp = relaxp
     net = vx.LearnedPreconditioner(backend=be)
     net.set_weights(load_weights(weights or DEFAULT_WEIGHTS))
     sim = vx.Simulation(p, case, backend=be)
@@ -347,6 +382,10 @@ def main():
                     help="skip the Jacobi-CG / learned flexible CG comparison on the last system")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
+    ap.add_argument("--relaxp", type=float, default=None,
+                    help="dam3d: pressure relaxation of the predictor / corrector split "
+                         "(default 0.2, the reference's; 1.0 drops the explicit pressure term "
+                         "of PREDICTION, which keeps the 3D dam break physical)")
     ap.add_argument("--precision-report", type=str, default="", metavar="STEPS",
                     help="instead of the benchmark, run the FP64 and the mixed-precision "
                          "simulations side by side and print the pressure-field L2 and Linf "
@@ -371,10 +410,12 @@ def main():
         stream_bandwidth(args.stream_bandwidth)
         return
     if args.precision_report:
-        precision_report([int(s) for s in args.precision_report.split(",")], args.scale)
+        precision_report([int(s) for s in args.precision_report.split(",")], args.scale,
+                         args.relaxp)
         return
     if args.ppe_report:
-        ppe_report([int(s) for s in args.ppe_report.split(",")], args.scale, args.lpc_weights)
+        ppe_report([int(s) for s in args.ppe_report.split(",")], args.scale, args.lpc_weights,
+                   args.relaxp)
         return
 
     import voxarmps_amd as vx
@@ -393,6 +434,8 @@ def main():
     else:
         case = vx.dam_break_3d_c1(scale=args.scale)
         p = vx.dam3d_params(case)
+        if args.relaxp is not None:
+            p.relaxp = args.relaxp
         precision = args.precision or "fp64"
     mixed = precision == "mixed"
     p.precision = vx._abi.PREC_MIXED if mixed else vx._abi.PREC_FP64
@@ -519,7 +562,7 @@ def main():
             "workload": workload,
             "particles": case.tp,
             "fluid": case.fp,
-            "re_s": "2.1 l0", "re_l": "4.0 l0", "dt": p.dt,
+            "re_s": "2.1 l0", "re_l": "4.0 l0", "dt": p.dt, "relaxp": p.relaxp,
             "ppe": (f"FP64 iterative refinement around FP32 Jacobi-PCG (eta {p.ir_eta:g}) to "
                     f"rel-res {p.cg_tol:g}" if mixed else f"Jacobi-PCG FP64 to rel-res {p.cg_tol:g}")
                    + (", warm start" if args.warm_start else ", cold start"),
