@@ -128,6 +128,33 @@ def _stage_cases():
     yield "2d-ref-gpu", j0, pg
 
 
+@pytest.mark.parametrize("name,case,p", list(_stage_cases())[:2], ids=["2d", "3d"])
+def test_prediction_single_fluid_bit_exact(hip, oracle, name, case, p):
+    """One fluid with Newtonian viscosity: every MEU is nu1 rho1, and the GPU
+    predictor then skips the neighbour's type (the harmonic mean of equal
+    values rounds back to the value). PREDICTION after VISCOSITY is still
+    bit-exact, walls and dummies included; after MEU is uploaded by hand the
+    general path runs and is bit-exact as well."""
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    for sim in (g, c):
+        random_state(sim, seed=11)
+    _sync_fields(c, g, ["U", "V", "W", "UNEW", "VNEW", "WNEW", "PNEW", "PHAT", "P"])
+    for st in ("PNUM", "VISCOSITY", "PREDICTION"):
+        getattr(g, st)()
+        getattr(c, st)()
+        _assert_same(g, c, FLOAT_OUT[st], f"{name} {st}")
+    rng = np.random.default_rng(3)
+    meu = rng.uniform(0.5e-3, 2e-3, case.tp)
+    for sim in (g, c):
+        sim.set("MEU", meu)
+        sim.PREDICTION()
+    _assert_same(g, c, FLOAT_OUT["PREDICTION"], f"{name} PREDICTION, MEU uploaded")
+    g.close()
+    c.close()
+
+
 @pytest.mark.parametrize("name,case,p", list(_stage_cases()), ids=lambda v: v if isinstance(v, str) else "")
 def test_stages_bit_exact(hip, oracle, name, case, p):
     case = case.copy()

@@ -470,6 +470,7 @@ int vox_set_params(vox_ctx* c, const vox_params* p)
     double pmax = c->p.pmax;
     c->p = *p;
     if (c->p.pmax <= 0) c->p.pmax = pmax;
+    c->meu_uniform = false;      // MEU was written under the previous parameters
     return VOX_OK;
 }
 
@@ -484,6 +485,7 @@ int vox_upload(vox_ctx* c, int field, const void* host)
     size_t bytes = (int_field(field) ? sizeof(int32_t) : sizeof(double)) * (size_t)c->tp;
     VOX_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->stream));
     VOX_HIP(hipStreamSynchronize(c->stream));
+    if (field == VOX_F_MEU) c->meu_uniform = false;
     if (field == VOX_F_PTYPE) {          // types never change during a run
         const int32_t* t = static_cast<const int32_t*>(host);
         long n2 = 0;

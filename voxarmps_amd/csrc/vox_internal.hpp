@@ -189,6 +189,8 @@ struct vox_ctx {
     double* f[VOX_F_COUNT] = {};
     int32_t* ptype = nullptr;
     long n_fluid2 = -1;            // PTYPE 2 particles in the uploaded types (-1: not known)
+    bool meu_uniform = false;      // MEU was last written by the Newtonian VISCOSITY with no
+                                   // PTYPE 2 particle: nu1 rho1 everywhere
     int32_t* bcon = nullptr;
 
     vox::DevList list[2];
@@ -367,6 +369,7 @@ int st_sps(vox_ctx* c);
 int st_vfraction(vox_ctx* c);
 int st_viscosity(vox_ctx* c);
 int st_prediction(vox_ctx* c);
+bool uniform_meu(const vox_ctx* c);
 int st_prediction32(vox_ctx* c);
 int st_bcon(vox_ctx* c);
 int st_phat(vox_ctx* c);

@@ -380,6 +380,7 @@ int st_viscosity(vox_ctx* c)
 {
     ProfScope ps(c, KC_VISC);
     const vox_params& p = c->p;
+    c->meu_uniform = (p.fluid2_type == 0 && c->n_fluid2 == 0);
     if (p.fluid2_type == 0) {
         k_visc_newton<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->ptype, p.nu1 * p.rho1,
                                                                  p.nu2 * p.rho2, c->f[VOX_F_MEU]);
@@ -420,6 +421,11 @@ __global__ void k_pack_pred(PredArgs a)
     a.rec[i] = r;
 }
 
+// kUniform: every particle has the same MEU m (Newtonian, no fluid 2) and
+// 2 m m / (m + m) rounds back to m, so the harmonic-mean branch of
+// functions.cu:2328-2337 gives the same NEU as the same-type branch: the
+// neighbour's type is not read (one gathered line less per neighbour).
+template <bool kUniform>
 __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
 {
     int i = xcd_tid();
@@ -454,7 +460,10 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
 #pragma unroll
     for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + 4 * h + u < cnt) ? jc[4 * h + u] : i;
 #pragma unroll
-    for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
+    for (int u = 0; u < U; ++u) {
+        rv[u] = a.rec[jj[u]];
+        tj[u] = kUniform ? ti : a.pt[jj[u]];
+    }
     // Branch-free per neighbour: every term is computed and a skipped entry
     // (self, padding, coincident particle) keeps each sum through a select, so
     // no gathered field is sunk into a branch behind a full memory wait.
@@ -470,8 +479,9 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
         // every quotient below equals the plain IEEE division (div_by); the
         // six divisions by D share one reciprocal
         const double inv_D = 1.0 / D;
-        const double hm = div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
-        const double NEU = (ti == tj[u]) ? div_by(mi, rhoN, inv_rhoN) : hm;
+        const double NEU = (kUniform || ti == tj[u])
+                               ? div_by(mi, rhoN, inv_rhoN)
+                               : div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
         const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
         const double wl = (a.re_l == a.re_s) ? wg : wfun_r(D, a.ktype, a.re_l, inv_re_l);
         const double dp = rj.pn - phi_;
@@ -504,6 +514,17 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
     }
 }
 
+// Every MEU equals nu1 rho1 (Newtonian viscosity and no PTYPE 2 particle:
+// functions.cu:2073-2085) and the harmonic mean 2 m m / (m + m) of that value
+// rounds back to it, so NEU does not depend on the neighbour's type.
+bool uniform_meu(const vox_ctx* c)
+{
+    if (!c->meu_uniform || c->p.fluid2_type != 0 || c->n_fluid2 != 0) return false;
+    const volatile double m = c->p.nu1 * c->p.rho1;
+    const volatile double hm = 2 * m * m / (m + m);
+    return hm == m;
+}
+
 int st_prediction(vox_ctx* c)
 {
     if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
@@ -527,7 +548,8 @@ int st_prediction(vox_ctx* c)
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = c->pred_rec;
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    k_prediction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    if (uniform_meu(c)) k_prediction<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    else k_prediction<false><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }

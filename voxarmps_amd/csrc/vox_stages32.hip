@@ -97,6 +97,9 @@ __global__ void k_pack_pred32(Pred32Args a)
     a.rec[i] = r;
 }
 
+// kUniform: every MEU is the same value (see uniform_meu): NEU does not
+// depend on the neighbour's type, which is then not read.
+template <bool kUniform>
 __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
 {
     const int i = xcd_tid();
@@ -122,7 +125,10 @@ __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
 #pragma unroll
         for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
-        for (int u = 0; u < U; ++u) { rv[u] = a.rec[jj[u]]; tj[u] = a.pt[jj[u]]; }
+        for (int u = 0; u < U; ++u) {
+            rv[u] = a.rec[jj[u]];
+            tj[u] = kUniform ? ti : a.pt[jj[u]];
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const Pred32 rj = rv[u];
@@ -130,8 +136,8 @@ __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
                         dz = qdiff(rj.z, me.z, a.q.sz);
             const float D = sqrtf(dx * dx + dy * dy + dz * dz);
             const bool use = (jj[u] != i) && (D > 0.0f);
-            const float hm = 2.0f * mi * rj.meu / (mi + rj.meu) / rhoN;
-            const float NEU = (ti == tj[u]) ? mi / rhoN : hm;
+            const float NEU = (kUniform || ti == tj[u]) ? mi / rhoN
+                                                        : 2.0f * mi * rj.meu / (mi + rj.meu) / rhoN;
             const float wg = wfun32(D, a.ktype, a.re_s);
             const float wl = (a.re_l == a.re_s) ? wg : wfun32(D, a.ktype, a.re_l);
             const float dp = rj.pn - phi_;
@@ -190,7 +196,8 @@ int st_prediction32(vox_ctx* c)
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = reinterpret_cast<Pred32*>(c->pred_rec);          // 64-byte slots hold 32-byte records
     k_pack_pred32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    k_prediction32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    if (uniform_meu(c)) k_prediction32<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    else k_prediction32<false><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
