@@ -67,13 +67,20 @@ def test_refined_solve_meets_fp64_residual(hip, precision, warm):
     q.precision = vx._abi.PREC_MIXED if precision == "mixed" else vx._abi.PREC_FP64
     q.warm_start = warm
     sim.set_params(q)
-    sim.MATRIX()                       # same FP64 system (plus the FP32 copy in mixed mode)
+    sim.MATRIX()                       # same system; mixed precision stores FP32 off-diagonals
     A2 = sim.ppe()
-    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+    for k in ("rowptr", "cols", "rhs", "ids"):
         assert np.array_equal(A[k], A2[k]), k
+    if precision == "mixed":
+        diag = np.zeros(A["vals"].shape[0], bool)
+        diag[A["rowptr"][:-1]] = True
+        assert np.array_equal(A2["vals"][diag], A["vals"][diag])
+        assert np.array_equal(A2["vals"][~diag], A["vals"][~diag].astype(np.float32).astype(np.float64))
+    else:
+        assert np.array_equal(A["vals"], A2["vals"])
     info = sim.CGM()
     x = sim.get("PNEW")[A["ids"]]
-    rel = _residual(A, x)
+    rel = _residual(A2, x)
     print(f"{precision} warm={warm}: iters {info.iterations} (cold FP64 {cold.iterations}), "
           f"relres gpu {info.relres:.3e} numpy {rel:.3e}")
     assert info.converged

@@ -410,11 +410,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_p(CgArgs<VT, MT> a)
 }
 
 // ---- outer FP64 refinement kernels ---------------------------------------------------------
+template <class MT>
 struct IrArgs {
     int nu, cap, npq;
     const int32_t* __restrict__ rowlen;
     const int32_t* __restrict__ acol;
-    const double* __restrict__ aval;
+    const MT* __restrict__ aval;
     const double* __restrict__ diag;
     const double* __restrict__ b;
     const double* __restrict__ x;
@@ -423,9 +424,11 @@ struct IrArgs {
     double* scal;
 };
 
-// res = b - A x in FP64 with the FP64 matrix; partials ||res||^2 and ||b||^2
-// per block.
-__global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
+// res = b - A x in FP64 arithmetic with the matrix values as stored (FP64, or
+// FP32 after a mixed-precision assembly); partials ||res||^2 and ||b||^2 per
+// block.
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs<MT> a)
 {
     __shared__ double lds[kBlock];
     double rr = 0.0, bb = 0.0;
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
         double acc = a.diag[r] * a.x[r];
         const int len = a.rowlen[r];
         const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
-        const double* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
+        const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
         constexpr int U = 8;
         int k = 0;
         for (; k + U <= len; k += U) {
@@ -443,14 +446,14 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 cc[u] = __builtin_nontemporal_load(&col[64L * (k + u)]);
-                vv[u] = __builtin_nontemporal_load(&val[64L * (k + u)]);
+                vv[u] = (double)__builtin_nontemporal_load(&val[64L * (k + u)]);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) xx[u] = a.x[cc[u]];
 #pragma unroll
             for (int u = 0; u < U; ++u) acc = acc + vv[u] * xx[u];
         }
-        for (; k < len; ++k) acc = acc + val[64L * k] * a.x[col[64L * k]];
+        for (; k < len; ++k) acc = acc + (double)val[64L * k] * a.x[col[64L * k]];
         const double b = a.b[r];
         const double rv = b - acc;
         a.res[r] = rv;
@@ -465,7 +468,8 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs a)
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_ir_residual_fin(IrArgs a)
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_ir_residual_fin(IrArgs<MT> a)
 {
     __shared__ double lds[kBlock];
     const double rr = sum_partials(PSLOT(a, P_RES), a.npq, lds);
@@ -657,8 +661,32 @@ static int solve_fp64(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
                   c->hflags[FLAG_CG_BREAK]);
 }
 
+// res = b - A x (FP64) over the matrix as stored, then the norms into the
+// scalar slots; FP64 values when the assembly kept them, else its FP32 values.
+static int ir_residual(vox_ctx* c, int npq)
+{
+    ProfScope ps(c, KC_IR);
+    if (c->aval64_ok) {
+        IrArgs<double> ir;
+        ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
+        ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval; ir.diag = c->adiag;
+        ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
+        k_ir_residual<double><<<npq, kBlock, 0, c->stream>>>(ir);
+        k_ir_residual_fin<double><<<1, kBlock, 0, c->stream>>>(ir);
+    } else {
+        IrArgs<float> ir;
+        ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
+        ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval32; ir.diag = c->adiag;
+        ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
+        k_ir_residual<float><<<npq, kBlock, 0, c->stream>>>(ir);
+        k_ir_residual_fin<float><<<1, kBlock, 0, c->stream>>>(ir);
+    }
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
 // Refinement driver: x = x0 (zero, or the previous pressure when warm);
-// repeat r = b - A x in FP64 on the FP64 matrix, stop when ||r|| <= tol ||b||,
+// repeat r = b - A x in FP64 on the stored matrix, stop when ||r|| <= tol ||b||,
 // else solve A d = r with CG<VT, MT> to a relative tolerance and x += d.
 // Mixed precision: VT = MT = float, each correction to max(eta, the remaining
 // factor). FP64 warm start: VT = MT = double, one correction normally reaches
@@ -668,11 +696,8 @@ static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x
                          VT* q, VT* z, double eta, bool warm, bool fcg, int maxit)
 {
     const int nu = c->nu;
-    IrArgs ir;
-    ir.nu = nu; ir.cap = c->a_cap; ir.npq = grid_for(nu > 0 ? nu : 1);
-    if (ir.npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
-    ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval; ir.diag = c->adiag;
-    ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
+    const int npq = grid_for(nu > 0 ? nu : 1);
+    if (npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
     if (warm && nu > 0) {
         k_gather_x0<<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->urow, c->f[VOX_F_P], c->cg_x);
         VOX_HIP(hipGetLastError());
@@ -687,12 +712,7 @@ static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x
         // needed, and the first inner solve takes b directly.
         const bool from_b = (outer == 0 && !warm);
         if (!from_b) {
-            {
-                ProfScope ps(c, KC_IR);
-                k_ir_residual<<<ir.npq, kBlock, 0, c->stream>>>(ir);
-                k_ir_residual_fin<<<1, kBlock, 0, c->stream>>>(ir);
-                VOX_HIP(hipGetLastError());
-            }
+            VOX_TRY(ir_residual(c, npq));
             VOX_TRY(read_flags(c));
             const double rn = c->hscal[S_RESN];
             bnorm = c->hscal[S_RHSN];
@@ -732,6 +752,9 @@ static int solve_plain(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 {
     VOX_TRY(fill_f64(c, c->f[VOX_F_PNEW], 0.0, c->tp));
     if (c->p.precision == VOX_PREC_MIXED) return solve_mixed(c, info, fcg, maxit);
+    // an FP64 solve of a system assembled in mixed precision solves its FP32
+    // values, which FP64 holds exactly
+    if (!c->aval64_ok) VOX_TRY(ppe_make_aval64(c));
     if (c->p.warm_start)
         return solve_refined<double, double>(c, info, (const double*)c->aval, c->cg_d, c->cg_e,
                                              c->cg_p, c->cg_q, c->cg_z, 0.0, true, fcg, maxit);

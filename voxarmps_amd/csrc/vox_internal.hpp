@@ -234,6 +234,7 @@ struct vox_ctx {
     // mixed precision: FP32 matrix values and inner-solve vectors
     float* aval32 = nullptr;       // same sliced layout as aval
     bool aval32_ok = false;        // aval32 holds the current assembly's values
+    bool aval64_ok = false;        // aval holds them in FP64 (not after a mixed assembly)
     float* cg_x32 = nullptr;
     float* cg_r32 = nullptr;
     float* cg_p32 = nullptr;
@@ -394,6 +395,7 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
 // operator for the context's assembled PPE
 int lpc_prepare(vox_ctx* c);
 int ppe_make_aval32(vox_ctx* c);   // FP32 copy of the assembled values (vox_ppe.hip)
+int ppe_make_aval64(vox_ctx* c);   // FP64 copy of FP32-assembled values
 void lpc_free_ctx(vox_ctx* c);
 
 }  // namespace vox

@@ -81,9 +81,28 @@ __device__ __forceinline__ void flush_chunk(const int32_t* rc, const double* rv,
     v2[2] = v2d{rv[s0 + 4], rv[s0 + 5]};
     v2[3] = v2d{rv[s0 + 6], rv[s0 + 7]};
 }
+// Mixed precision: the staging values are the FP64 values rounded to FP32
+// (one 32-byte store per chunk).
+__device__ __forceinline__ void flush_chunk(const int32_t* rc, const double* rv, int c,
+                                            int32_t* __restrict__ tcol, float* __restrict__ tval,
+                                            long rb)
+{
+    const int s0 = (c & 1) * 8;
+    const long o = rb + ((long)c << 9);
+    v4i* c4 = reinterpret_cast<v4i*>(tcol + o);
+    c4[0] = v4i{rc[s0 + 0], rc[s0 + 1], rc[s0 + 2], rc[s0 + 3]};
+    c4[1] = v4i{rc[s0 + 4], rc[s0 + 5], rc[s0 + 6], rc[s0 + 7]};
+    v4f* f4 = reinterpret_cast<v4f*>(tval + o);
+    f4[0] = v4f{(float)rv[s0 + 0], (float)rv[s0 + 1], (float)rv[s0 + 2], (float)rv[s0 + 3]};
+    f4[1] = v4f{(float)rv[s0 + 4], (float)rv[s0 + 5], (float)rv[s0 + 6], (float)rv[s0 + 7]};
+}
 
+// SV: staging value type, double, or float for mixed precision (the values
+// are computed in FP64 either way; the diagonal sums them in FP64).
+template <class SV>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 {
+    SV* __restrict__ tval = reinterpret_cast<SV*>(a.tval);
     __shared__ int32_t ring_c[kBlock * kRing];
     __shared__ double ring_v[kBlock * kRing];
     int i = xcd_tid();
@@ -145,9 +164,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
             diag = use ? diag + val : diag;
         }
         // at most 8 outputs per step: at most one group of 8 completed
-        if ((k0 >> 3) != (k >> 3)) flush_chunk(rc, rv, k0 >> 3, a.tcol, a.tval, rb);
+        if ((k0 >> 3) != (k >> 3)) flush_chunk(rc, rv, k0 >> 3, a.tcol, tval, rb);
     }
-    if (k & 7) flush_chunk(rc, rv, k >> 3, a.tcol, a.tval, rb);   // the last, partial group
+    if (k & 7) flush_chunk(rc, rv, k >> 3, a.tcol, tval, rb);   // the last, partial group
     diag += a.dshift;
     a.adiag[r] = diag;
     a.rowlen[r] = k;
@@ -158,10 +177,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 // group g of 64 rows is one contiguous block there (lane t's 8 entries at
 // offset 8t), and entry 8g + u of the 64 rows is one contiguous block here, so
 // every load and store of a wave is contiguous. One thread per row.
+// SV = double: FP64 staging to aval (and aval32 when given, for the learned
+// preconditioner); SV = float: FP32 staging to aval32 only (mixed precision).
+template <class SV>
 __global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
                                                           const int32_t* __restrict__ rowlen,
                                                           const int32_t* __restrict__ tcol,
-                                                          const double* __restrict__ tval,
+                                                          const SV* __restrict__ tval,
                                                           int32_t* __restrict__ acol,
                                                           double* __restrict__ aval,
                                                           float* __restrict__ aval32)
@@ -173,26 +195,49 @@ __global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
     const int len = rowlen[r];
     for (int g = 0; (g << 3) < len; ++g) {
         int32_t cc[8];
-        double vv[8];
+        SV vv[8];
         {
             const v4i* cp = reinterpret_cast<const v4i*>(tcol + rb + ((long)g << 9));
             const v4i c0 = cp[0], c1 = cp[1];
             cc[0] = c0.x; cc[1] = c0.y; cc[2] = c0.z; cc[3] = c0.w;
             cc[4] = c1.x; cc[5] = c1.y; cc[6] = c1.z; cc[7] = c1.w;
-            const v2d* vp = reinterpret_cast<const v2d*>(tval + rb + ((long)g << 9));
+            if constexpr (sizeof(SV) == 8) {
+                const v2d* vp = reinterpret_cast<const v2d*>(tval + rb + ((long)g << 9));
 #pragma unroll
-            for (int h = 0; h < 4; ++h) { const v2d t = vp[h]; vv[2 * h] = t.x; vv[2 * h + 1] = t.y; }
+                for (int h = 0; h < 4; ++h) { const v2d t = vp[h]; vv[2 * h] = t.x; vv[2 * h + 1] = t.y; }
+            } else {
+                const v4f* vp = reinterpret_cast<const v4f*>(tval + rb + ((long)g << 9));
+                const v4f f0 = vp[0], f1 = vp[1];
+                vv[0] = f0.x; vv[1] = f0.y; vv[2] = f0.z; vv[3] = f0.w;
+                vv[4] = f1.x; vv[5] = f1.y; vv[6] = f1.z; vv[7] = f1.w;
+            }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int kk = (g << 3) + u;
             if (kk < len) {
                 acol[ob + 64L * kk] = cc[u];
-                aval[ob + 64L * kk] = vv[u];
-                if (aval32) aval32[ob + 64L * kk] = (float)vv[u];   // mixed precision
+                if constexpr (sizeof(SV) == 8) {
+                    aval[ob + 64L * kk] = vv[u];
+                    if (aval32) aval32[ob + 64L * kk] = (float)vv[u];
+                } else {
+                    aval32[ob + 64L * kk] = vv[u];
+                }
             }
         }
     }
+}
+
+// FP64 copy of FP32-stored values (a solve in FP64 of a system assembled in
+// mixed precision: the FP32 values are exact in FP64).
+__global__ void k_aval64(int nu, int cap, const int32_t* __restrict__ rowlen,
+                         const float* __restrict__ aval32, double* __restrict__ aval)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nu) return;
+    const long ob = mx_index(r, 0, cap);
+    const int len = rowlen[r];
+    for (int k = 0; k < len; ++k) aval[ob + 64L * k] = (double)aval32[ob + 64L * k];
 }
 
 // FP32 copy of the off-diagonal values for the mixed-precision inner solves:
@@ -316,10 +361,12 @@ int ppe_assemble(vox_ctx* c)
         c->aval32_ok = false;
         c->a_cap = L.cap;
     }
-    // FP32 values: the mixed-precision inner solves and the learned
-    // preconditioner's network read them
-    const bool mixed = (c->p.precision == VOX_PREC_MIXED) ||
-                       (c->p.solver == VOX_SOLVER_LEARNED_FCG && c->lpc);
+    // Mixed precision stores the off-diagonal values in FP32 only (computed in
+    // FP64 and rounded once; diagonal and right-hand side stay FP64). The FP64
+    // mode stores FP64 values, plus an FP32 copy for the learned
+    // preconditioner's network.
+    const bool fp32_only = (c->p.precision == VOX_PREC_MIXED);
+    const bool mixed = fp32_only || (c->p.solver == VOX_SOLVER_LEARNED_FCG && c->lpc);
     k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
     VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
     k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
@@ -348,20 +395,28 @@ int ppe_assemble(vox_ctx* c)
     a.source = c->f[VOX_F_SOURCE];
     a.rec = c->pos_rec;
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
-    k_assemble<<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    if (fp32_only) k_assemble<float><<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    else k_assemble<double><<<grid_for(n), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
     c->aval32_ok = false;
+    c->aval64_ok = false;
     if (mixed && !c->aval32)
         VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
     if (c->nu > 0) {
-        k_asm_transpose<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen,
-                                                                   c->tcol, c->tval, c->acol,
-                                                                   c->aval, mixed ? c->aval32 : nullptr);
+        if (fp32_only)
+            k_asm_transpose<float><<<grid_for(c->nu), kBlock, 0, c->stream>>>(
+                c->nu, c->a_cap, c->rowlen, c->tcol, reinterpret_cast<const float*>(c->tval),
+                c->acol, c->aval, c->aval32);
+        else
+            k_asm_transpose<double><<<grid_for(c->nu), kBlock, 0, c->stream>>>(
+                c->nu, c->a_cap, c->rowlen, c->tcol, c->tval, c->acol, c->aval,
+                mixed ? c->aval32 : nullptr);
         VOX_HIP(hipGetLastError());
     }
     c->aval32_ok = mixed;
+    c->aval64_ok = !fp32_only;
     if (p.srcopt) {
         SrcArgs s;
         s.n = n; s.dim = p.dim; s.re = p.re_l; s.n0s = c->n0s; s.dt = p.dt;
@@ -415,8 +470,20 @@ int st_pressure_wc(vox_ctx* c)
     return VOX_OK;
 }
 
+int ppe_make_aval64(vox_ctx* c)
+{
+    if (c->nu > 0) {
+        k_aval64<<<grid_for(c->nu), kBlock, 0, c->stream>>>(c->nu, c->a_cap, c->rowlen, c->aval32,
+                                                           c->aval);
+        VOX_HIP(hipGetLastError());
+    }
+    c->aval64_ok = true;
+    return VOX_OK;
+}
+
 int ppe_make_aval32(vox_ctx* c)
 {
+    if (!c->aval64_ok) return fail(VOX_ERR_STATE, "no assembled FP64 values to round");
     if (!c->aval32)
         VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
     if (c->nu > 0) {
@@ -449,7 +516,13 @@ int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, i
     std::vector<int32_t> urow(nu > 0 ? nu : 1);
     if (nu > 0) {
         VOX_HIP(hipMemcpy(acol.data(), c->acol, sizeof(int32_t) * tot, hipMemcpyDeviceToHost));
-        VOX_HIP(hipMemcpy(aval.data(), c->aval, sizeof(double) * tot, hipMemcpyDeviceToHost));
+        if (c->aval64_ok) {
+            VOX_HIP(hipMemcpy(aval.data(), c->aval, sizeof(double) * tot, hipMemcpyDeviceToHost));
+        } else {            // mixed precision: the values are stored in FP32
+            std::vector<float> a32(tot);
+            VOX_HIP(hipMemcpy(a32.data(), c->aval32, sizeof(float) * tot, hipMemcpyDeviceToHost));
+            for (long q = 0; q < tot; ++q) aval[q] = (double)a32[q];
+        }
         VOX_HIP(hipMemcpy(diag.data(), c->adiag, sizeof(double) * nu, hipMemcpyDeviceToHost));
         VOX_HIP(hipMemcpy(b.data(), c->rhs, sizeof(double) * nu, hipMemcpyDeviceToHost));
         VOX_HIP(hipMemcpy(urow.data(), c->urow, sizeof(int32_t) * nu, hipMemcpyDeviceToHost));
@@ -539,6 +612,7 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
     VOX_HIP(hipMemcpy(c->uidx, uidx.data(), sizeof(int32_t) * c->tp, hipMemcpyHostToDevice));
     c->nu = nu;
     c->aval32_ok = false;
+    c->aval64_ok = true;
     return VOX_OK;
 }
 
