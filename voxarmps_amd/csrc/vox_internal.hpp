@@ -372,6 +372,10 @@ int st_viscosity(vox_ctx* c);
 int st_prediction(vox_ctx* c);
 bool uniform_meu(const vox_ctx* c);
 int st_prediction32(vox_ctx* c);
+int st_phat32(vox_ctx* c);          // the FP32 twins of the mixed precision mode
+int st_sps32(vox_ctx* c);
+int st_visc_hb32(vox_ctx* c);
+int st_vfraction32(vox_ctx* c);
 int st_bcon(vox_ctx* c);
 int st_phat(vox_ctx* c);
 int st_pressgrad(vox_ctx* c);

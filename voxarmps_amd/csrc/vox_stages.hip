@@ -266,6 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_sps2(SpsArgs a)
 
 int st_sps(vox_ctx* c)
 {
+    if (c->p.precision == VOX_PREC_MIXED) return st_sps32(c);
     ProfScope ps(c, KC_SPS);
     SpsArgs a;
     a.n = c->tp; a.first = c->gp; a.dim = c->p.dim; a.ktype = c->p.ktype;
@@ -326,6 +327,7 @@ int st_vfraction(vox_ctx* c)
     // Without fluid-2 particles every sum of fluid-2 weights is +0, so each C
     // is 0 / s1 = +0 (or the s1 == 0 case, also +0): the same bits as the walk.
     if (c->n_fluid2 == 0) return fill_f64(c, c->f[VOX_F_C], 0.0, c->tp);
+    if (c->p.precision == VOX_PREC_MIXED) return st_vfraction32(c);
     k_vfraction<<<grid_for(c->tp), kBlock, 0, c->stream>>>(
         c->tp, c->p.dim, c->p.ktype, c->p.re_s, view(c, VOX_LIST_S), c->ptype, c->f[VOX_F_X],
         c->f[VOX_F_Y], c->f[VOX_F_Z], c->f[VOX_F_C]);
@@ -384,6 +386,8 @@ int st_viscosity(vox_ctx* c)
     if (p.fluid2_type == 0) {
         k_visc_newton<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->ptype, p.nu1 * p.rho1,
                                                                  p.nu2 * p.rho2, c->f[VOX_F_MEU]);
+    } else if (p.precision == VOX_PREC_MIXED) {
+        VOX_TRY(st_visc_hb32(c));
     } else {
         HbArgs a;
         a.n = c->tp; a.dim = p.dim; a.ktype = p.ktype; a.re = p.re_s; a.n0 = c->n0s;
@@ -623,6 +627,7 @@ __global__ void k_phat(int n, ListView L, const double* __restrict__ pn, double*
 
 int st_phat(vox_ctx* c)
 {
+    if (c->p.precision == VOX_PREC_MIXED) return st_phat32(c);
     ProfScope ps(c, KC_PHAT);
     k_phat<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, view(c, VOX_LIST_S),
                                                      c->f[VOX_F_PNEW], c->f[VOX_F_PHAT]);

@@ -1,6 +1,7 @@
-// vox_stages32.hip — FP32 twins of the two velocity stages for the mixed
-// precision mode: the predictor (viscous + explicit pressure terms) and the
-// pressure-gradient correction.
+// vox_stages32.hip — FP32 twins of the particle stages for the mixed
+// precision mode: the predictor (viscous + explicit pressure terms), the
+// pressure-gradient correction, p-hat, the SPS-LES eddy viscosity and stress
+// divergence, the Herschel-Bulkley viscosity and the volume fraction.
 //
 // The particle state stays FP64: positions are integrated in FP64 and the
 // particle number densities, which feed the PPE source (n* - n0)/n0 through a
@@ -21,7 +22,8 @@ namespace vox {
 
 struct __align__(16) Pred32 { uint32_t x, y, z; float u, v, w, pn, meu; };   // 32 B
 struct __align__(8) Grad32 { uint32_t x, y, z; float pn, ph, pad; };          // 24 B
-static_assert(sizeof(Pred32) == 32 && sizeof(Grad32) == 24, "record sizes");
+struct __align__(16) Pos32 { uint32_t x, y, z; float f; };                    // 16 B
+static_assert(sizeof(Pred32) == 32 && sizeof(Grad32) == 24 && sizeof(Pos32) == 16, "record sizes");
 
 // Fixed-point frame: origin and 2^32/extent per axis (packing), extent/2^32
 // per axis (differences).
@@ -294,6 +296,341 @@ int st_pressgrad32(vox_ctx* c)
     k_pack_grad32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     const int m = c->tp - a.first;
     if (m > 0) k_pressgrad32<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+}  // namespace vox
+
+namespace vox {
+
+// ---- PHATCALC (functions.cu:207-224) ---------------------------------------------------------
+// The minimum of the FP32-rounded pressures: rounding is monotone, so this is
+// the FP64 p-hat rounded to FP32 (1e15, the reference's start value, when the
+// row is empty).
+__global__ void k_pack_p32(int n, const double* __restrict__ pn, float* __restrict__ p32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p32[i] = (float)pn[i];
+}
+
+__global__ void k_phat32(int n, ListView L, const float* __restrict__ p32, double* __restrict__ ph)
+{
+    const int i = xcd_tid();
+    if (i >= n) return;
+    float mn = 3.0e38f;
+    const int cnt = L.cnt[i];
+    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    for (int g = 0; (g << 3) < cnt; ++g) {
+        int32_t jc[8];
+        float v[8];
+        load_chunk8(rowp, g, jc);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) jc[e] = ((g << 3) + e < cnt) ? jc[e] : i;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = p32[jc[e]];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) mn = ((g << 3) + e < cnt && v[e] < mn) ? v[e] : mn;
+    }
+    ph[i] = cnt > 0 ? (double)mn : 999999999999999.0;
+}
+
+int st_phat32(vox_ctx* c)
+{
+    ProfScope ps(c, KC_PHAT);
+    float* p32 = reinterpret_cast<float*>(c->scratch_d);      // tp floats in tp doubles
+    k_pack_p32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->f[VOX_F_PNEW], p32);
+    k_phat32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, view(c, VOX_LIST_S), p32,
+                                                        c->f[VOX_F_PHAT]);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- strain rate over list S (SPS functions.cu:1743-1788, VISCOSITY 2096-2140) --------------
+struct Strain32 { float s11, s12, s22, s13, s23, s33; };
+
+struct Vel32Args {
+    int n, first, dim, ktype;
+    Quant q;
+    float re, n0, dmin, csdl2;
+    bool use_dmin;
+    ListView L;
+    const double *x, *y, *z, *u, *v, *w;
+    Pred32* rec;
+};
+
+__global__ void k_pack_vel32(Vel32Args a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    Pred32 r;
+    r.x = qpos(a.x[i], a.q.ox, a.q.ix);
+    r.y = qpos(a.y[i], a.q.oy, a.q.iy);
+    r.z = (a.dim == 3) ? qpos(a.z[i], a.q.oz, a.q.iz) : 0u;
+    r.u = (float)a.u[i]; r.v = (float)a.v[i]; r.w = (a.dim == 3) ? (float)a.w[i] : 0.0f;
+    r.pn = 0.0f; r.meu = 0.0f;
+    a.rec[i] = r;
+}
+
+// Velocity-gradient sums of row i in FP32; the same terms and row order as
+// the FP64 strain_row, branch-free (8 records in flight).
+__device__ __forceinline__ Strain32 strain_row32(int i, const Vel32Args& a)
+{
+    const Pred32 me = a.rec[i];
+    float s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
+    const int cnt = a.L.cnt[i];
+    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    for (int l0 = 0; l0 < cnt; l0 += 8) {
+        int jj[8];
+        Pred32 rv[8];
+        load_chunk8(row, l0 >> 3, jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rv[u] = a.rec[jj[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const Pred32 rj = rv[u];
+            const float dx = qdiff(rj.x, me.x, a.q.sx), dy = qdiff(rj.y, me.y, a.q.sy),
+                        dz = qdiff(rj.z, me.z, a.q.sz);
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+            const bool use = (jj[u] != i) && (d > 0.0f) && (!a.use_dmin || d > a.dmin);
+            const float g = wfun32(d, a.ktype, a.re) / d / d;
+            const float du = rj.u - me.u, dv = rj.v - me.v, dw = rj.w - me.w;
+            s1 = use ? s1 + du * dx * g : s1;
+            s2 = use ? s2 + dv * dy * g : s2;
+            s3 = use ? s3 + du * dy * g : s3;
+            s4 = use ? s4 + dv * dx * g : s4;
+            s5 = use ? s5 + dw * dz * g : s5;
+            s6 = use ? s6 + du * dz * g : s6;
+            s7 = use ? s7 + dv * dz * g : s7;
+            s8 = use ? s8 + dw * dx * g : s8;
+            s9 = use ? s9 + dw * dy * g : s9;
+        }
+    }
+    const float c2 = 2.0f / a.n0;
+    const float Uxx = c2 * s1, Uyy = c2 * s2, Uxy = c2 * s3, Uyx = c2 * s4;
+    Strain32 S;
+    S.s12 = 0.5f * (Uxy + Uyx);
+    S.s11 = 0.5f * (Uxx + Uxx);
+    S.s22 = 0.5f * (Uyy + Uyy);
+    S.s13 = S.s23 = S.s33 = 0.0f;
+    if (a.dim == 3) {
+        const float Uzz = c2 * s5, Uxz = c2 * s6, Uyz = c2 * s7, Uzx = c2 * s8, Uzy = c2 * s9;
+        S.s33 = 0.5f * (Uzz + Uzz);
+        S.s13 = 0.5f * (Uxz + Uzx);
+        S.s23 = 0.5f * (Uyz + Uzy);
+    }
+    return S;
+}
+
+// ---- SPS (functions.cu:1719-1841) --------------------------------------------------------------
+struct Sps32Args {
+    Vel32Args v;
+    double *s11, *s12, *s22, *s13, *s23, *s33, *neut, *t1, *t2, *t3;
+    Pos32* nrec;                 // positions + NEUt, for the second pass
+};
+
+__global__ __launch_bounds__(kBlock) void k_sps1_32(Sps32Args a)
+{
+    const int i = a.v.first + xcd_tid();
+    if (i >= a.v.n) return;
+    const Strain32 S = strain_row32(i, a.v);
+    a.s11[i] = S.s11; a.s12[i] = S.s12; a.s22[i] = S.s22;
+    a.s13[i] = S.s13; a.s23[i] = S.s23; a.s33[i] = S.s33;
+    float s = sqrtf(2 * S.s11 * S.s11) + 2 * sqrtf(2 * S.s12 * S.s12) + sqrtf(2 * S.s22 * S.s22);
+    if (a.v.dim == 3)
+        s = s + sqrtf(2 * S.s33 * S.s33) + 2 * sqrtf(2 * S.s13 * S.s13) + 2 * sqrtf(2 * S.s23 * S.s23);
+    a.neut[i] = (double)(a.v.csdl2 * s);
+}
+
+__global__ void k_pack_neut32(Sps32Args a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.v.n) return;
+    const Pred32 r = a.v.rec[i];
+    a.nrec[i] = Pos32{r.x, r.y, r.z, (float)a.neut[i]};
+}
+
+__global__ __launch_bounds__(kBlock) void k_sps2_32(Sps32Args a)
+{
+    const int i = a.v.first + xcd_tid();
+    if (i >= a.v.n) return;
+    const Pos32 me = a.nrec[i];
+    float s1 = 0, s2 = 0, s3 = 0;
+    const int cnt = a.v.L.cnt[i];
+    const int32_t* __restrict__ row = a.v.L.ids + sl_index(i, 0, a.v.L.cap);
+    for (int l0 = 0; l0 < cnt; l0 += 8) {
+        int jj[8];
+        Pos32 rv[8];
+        load_chunk8(row, l0 >> 3, jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rv[u] = a.nrec[jj[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const Pos32 rj = rv[u];
+            const float dx = qdiff(rj.x, me.x, a.v.q.sx), dy = qdiff(rj.y, me.y, a.v.q.sy),
+                        dz = qdiff(rj.z, me.z, a.v.q.sz);
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+            const bool use = (jj[u] != i) && (d > 0.0f) && (d > a.v.dmin);
+            const float g = wfun32(d, a.v.ktype, a.v.re) / d / d;
+            const float dn = rj.f - me.f;
+            s1 = use ? s1 + dn * dx * g : s1;
+            s2 = use ? s2 + dn * dy * g : s2;
+            s3 = use ? s3 + dn * dz * g : s3;
+        }
+    }
+    const float c2 = 2.0f / a.v.n0;
+    const float s11 = (float)a.s11[i], s12 = (float)a.s12[i], s22 = (float)a.s22[i];
+    if (a.v.dim == 3) {
+        const float s13 = (float)a.s13[i], s23 = (float)a.s23[i], s33 = (float)a.s33[i];
+        a.t1[i] = c2 * (2 * s11 * s1 + 2 * s12 * s2 + 2 * s13 * s3);
+        a.t2[i] = c2 * (2 * s12 * s1 + 2 * s22 * s2 + 2 * s23 * s3);
+        a.t3[i] = c2 * (2 * s13 * s1 + 2 * s23 * s2 + 2 * s33 * s3);
+    } else {
+        a.t1[i] = c2 * (2 * s11 * s1 + 2 * s12 * s2);
+        a.t2[i] = c2 * (2 * s12 * s1 + 2 * s22 * s2);
+    }
+}
+
+static Vel32Args vel_args(vox_ctx* c, bool use_dmin)
+{
+    const vox_params& p = c->p;
+    Vel32Args v;
+    v.n = c->tp; v.first = c->gp; v.dim = p.dim; v.ktype = p.ktype; v.q = make_quant(p);
+    v.re = (float)p.re_s; v.n0 = (float)c->n0s; v.dmin = (float)(p.coll / 10);
+    v.csdl2 = (float)((p.cs * p.dl) * (p.cs * p.dl));
+    v.use_dmin = use_dmin;
+    v.L = view(c, VOX_LIST_S);
+    v.x = c->f[VOX_F_X]; v.y = c->f[VOX_F_Y]; v.z = c->f[VOX_F_Z];
+    v.u = c->f[VOX_F_UNEW]; v.v = c->f[VOX_F_VNEW]; v.w = c->f[VOX_F_WNEW];
+    v.rec = reinterpret_cast<Pred32*>(c->pred_rec);          // 64-byte slots hold 32-byte records
+    return v;
+}
+
+int st_sps32(vox_ctx* c)
+{
+    ProfScope ps(c, KC_SPS);
+    Sps32Args a;
+    a.v = vel_args(c, true);
+    a.s11 = c->sps_s[0]; a.s12 = c->sps_s[1]; a.s22 = c->sps_s[2];
+    a.s13 = c->sps_s[3]; a.s23 = c->sps_s[4]; a.s33 = c->sps_s[5];
+    a.neut = c->f[VOX_F_NEUT];
+    a.t1 = c->f[VOX_F_TURB1]; a.t2 = c->f[VOX_F_TURB2]; a.t3 = c->f[VOX_F_TURB3];
+    a.nrec = reinterpret_cast<Pos32*>(c->pos_rec);           // 32-byte slots hold 16-byte records
+    const int m = c->tp - c->gp;
+    if (m <= 0) return VOX_OK;
+    k_pack_vel32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a.v);
+    k_sps1_32<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    k_pack_neut32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    k_sps2_32<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- Herschel-Bulkley VISCOSITY (functions.cu:2053-2184) ---------------------------------------
+struct Hb32Args {
+    Vel32Args v;
+    const int32_t* pt;
+    const double* phat;
+    double* meu;
+    double m1;
+    float cohes, phi_param, meu0, npl;
+};
+
+__global__ __launch_bounds__(kBlock) void k_visc_hb32(Hb32Args a)
+{
+    const int i = xcd_tid();
+    if (i >= a.v.n) return;
+    if (a.pt[i] == 1) { a.meu[i] = a.m1; return; }
+    const Strain32 S = strain_row32(i, a.v);
+    // phi reads C[I] with the caller's I == 0 (main.cu:211): phi = 0 (2154-2155)
+    const float cz = 0.0f;
+    float phi = (cz - 0.25f) * a.phi_param / (1 - 0.25f);
+    if (cz < 0.25f) phi = 0;
+    const float ys = a.cohes * cosf(phi) + (float)a.phat[i] * sinf(phi);
+    float II;
+    if (a.v.dim == 3)
+        II = fabsf((S.s11 * S.s22 * S.s33 + S.s12 * S.s23 * S.s13 + S.s13 * S.s12 * S.s23) -
+                   (S.s13 * S.s22 * S.s13 + S.s11 * S.s23 * S.s23 + S.s12 * S.s12 * S.s33));
+    else
+        II = fabsf(S.s11 * S.s22 - S.s12 * S.s12);
+    float m = ys / 2 / sqrtf(II) + a.meu0 * powf(4 * II, (a.npl - 1) / 2);
+    if (II == 0 || m > 200) m = 200;
+    a.meu[i] = (double)m;
+}
+
+int st_visc_hb32(vox_ctx* c)
+{
+    const vox_params& p = c->p;
+    Hb32Args a;
+    a.v = vel_args(c, false);
+    a.v.first = 0;
+    a.pt = c->ptype; a.phat = c->f[VOX_F_PHAT]; a.meu = c->f[VOX_F_MEU];
+    a.m1 = p.nu1 * p.rho1; a.cohes = (float)p.cohes; a.phi_param = (float)p.phi;
+    a.meu0 = (float)p.meu0; a.npl = (float)p.n_pl;
+    k_pack_vel32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a.v);
+    k_visc_hb32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+// ---- V_FRACTION, method 2 (functions.cu:2604-2634) ----------------------------------------------
+__global__ void k_pack_type32(int n, int dim, Quant q, const double* __restrict__ x,
+                              const double* __restrict__ y, const double* __restrict__ z,
+                              const int32_t* __restrict__ pt, Pos32* __restrict__ rec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    rec[i] = Pos32{qpos(x[i], q.ox, q.ix), qpos(y[i], q.oy, q.iy),
+                   dim == 3 ? qpos(z[i], q.oz, q.iz) : 0u, (float)pt[i]};
+}
+
+__global__ __launch_bounds__(kBlock) void k_vfraction32(int n, int ktype, float re, Quant q,
+                                                        ListView L, const Pos32* __restrict__ rec,
+                                                        double* __restrict__ C)
+{
+    const int i = xcd_tid();
+    if (i >= n) return;
+    const Pos32 me = rec[i];
+    float s1 = 0, s2 = 0;
+    const int cnt = L.cnt[i];
+    const int32_t* __restrict__ row = L.ids + sl_index(i, 0, L.cap);
+    for (int l0 = 0; l0 < cnt; l0 += 8) {
+        int jj[8];
+        Pos32 rv[8];
+        load_chunk8(row, l0 >> 3, jj);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) rv[u] = rec[jj[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const Pos32 rj = rv[u];
+            const float dx = qdiff(rj.x, me.x, q.sx), dy = qdiff(rj.y, me.y, q.sy),
+                        dz = qdiff(rj.z, me.z, q.sz);
+            const float wv = wfun32(sqrtf(dx * dx + dy * dy + dz * dz), ktype, re);
+            const bool use = (jj[u] != i) && (rj.f > 0.0f);
+            s1 = use ? s1 + wv : s1;
+            s2 = (use && rj.f == 2.0f) ? s2 + wv : s2;
+        }
+    }
+    C[i] = (s1 == 0.0f) ? 0.0 : (double)(s2 / s1);
+}
+
+int st_vfraction32(vox_ctx* c)
+{
+    const vox_params& p = c->p;
+    Pos32* rec = reinterpret_cast<Pos32*>(c->pos_rec);
+    const Quant q = make_quant(p);
+    k_pack_type32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, p.dim, q, c->f[VOX_F_X],
+                                                             c->f[VOX_F_Y], c->f[VOX_F_Z],
+                                                             c->ptype, rec);
+    k_vfraction32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, p.ktype, (float)p.re_s, q,
+                                                             view(c, VOX_LIST_S), rec,
+                                                             c->f[VOX_F_C]);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
