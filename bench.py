@@ -382,6 +382,8 @@ def main():
                     help="skip the Jacobi-CG / learned flexible CG comparison on the last system")
     ap.add_argument("--warm-start", action="store_true",
                     help="start each PPE solve from the previous step's pressure")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (neighbour walks, predictor and product variants)")
     ap.add_argument("--relaxp", type=float, default=None,
                     help="dam3d: pressure relaxation of the predictor / corrector split "
                          "(default 0.2, the reference's; 1.0 drops the explicit pressure term "
@@ -455,8 +457,9 @@ def main():
     if ensemble:
         sim.set_regions(region, boxes)
     sim.init()
-    if ensemble:
-        modes = sim.autotune(reps=3)
+    if not args.no_autotune:
+        sim.autotune(reps=3)
+        modes = sim.tuning()
     print(f"# init done: {case.tp} particles", file=sys.stderr, flush=True)
     if args.warmup:
         sim.step(args.warmup)
@@ -570,7 +573,9 @@ def main():
             "solver": ("flexible CG, learned preconditioner (Jacobi-CG fallback: "
                        f"{fallbacks} fallbacks)") if args.solver == "learned" else "Jacobi-PCG",
             "parallelism": "1 GPU",
-            "search_walks": modes,
+            "tuning": None if modes is None else {
+                "walk_s": modes[0], "walk_l": modes[1], "predictor_variant": modes[2],
+                "product_variant": modes[3]},
             "solo_runs": solo,
         },
         "roofline": {

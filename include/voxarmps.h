@@ -277,9 +277,17 @@ int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const doub
 /* Tile-shape autotuner (no counterpart in the reference): times both
  * neighbour-search walks (1: one thread per particle, 2: one wave per grid
  * cell) over `reps` builds on the current positions and keeps the faster per
- * list; modes[2] receives the choice for lists S and L. The lists themselves do
- * not depend on it. Requires vox_init. */
+ * list (modes[2] receives the choice for lists S and L); then runs the step's
+ * stages up to MATRIX once and times the predictor's variants (FP64: records
+ * gathered per group 4, 8 or 2 with 256 threads per block, 4 or 8 with 128)
+ * and the PPE product's (8, 4 or 16 entries per load batch). Every variant
+ * gives the same bits; the context keeps the fastest. Requires vox_init. */
 int vox_autotune(vox_ctx* ctx, int reps, int32_t* modes);
+/* The context's tuning (no counterpart in the reference): choice[4] = walk of
+ * list S, walk of list L (0 = heuristic), predictor variant (0..4), product
+ * variant (0..2); vox_set_tuning applies a cached choice without timing. */
+int vox_get_tuning(vox_ctx* ctx, int32_t* choice);
+int vox_set_tuning(vox_ctx* ctx, const int32_t* choice);
 
 /* Reference densities; vox_set_n0 lets a caller impose its own. */
 int vox_get_n0(vox_ctx* ctx, double* n0_s, double* n0_l);

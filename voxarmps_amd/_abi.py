@@ -137,6 +137,8 @@ PROTOTYPES = {
     "vox_get_n0": (C.c_int, [_P, _dp, _dp]),
     "vox_set_regions": (C.c_int, [_P, C.c_int, _i32p, _dp]),
     "vox_autotune": (C.c_int, [_P, C.c_int, _i32p]),
+    "vox_get_tuning": (C.c_int, [_P, _i32p]),
+    "vox_set_tuning": (C.c_int, [_P, _i32p]),
     "vox_set_n0": (C.c_int, [_P, C.c_double, C.c_double]),
     "vox_get_neighbors": (C.c_int, [_P, C.c_int, _i32p, _i32p, C.c_int]),
     "vox_get_neighbors_slot": (C.c_int, [_P, C.c_int, _i32p, C.c_int]),

@@ -144,6 +144,63 @@ static int need_init(vox_ctx* c)
 
 using namespace vox;
 
+namespace vox {
+
+// Kernel variants of the autotuner: every variant gives the same bits, so
+// the choice is by time only. The step's stages up to MATRIX run once on the
+// current state (each field they write is rewritten by the next step before
+// it is read; the pressure is not touched), then each predictor variant
+// (FP64 path) and each PPE product variant is timed over `reps` launches.
+int autotune_kernels(vox_ctx* c, int reps)
+{
+    if (reps < 1) reps = 1;
+    const bool wc = (c->p.method == 3);
+    if (!wc) VOX_TRY(st_pnd(c, 0, 1));
+    VOX_TRY(build_neighbors(c));
+    if (c->p.turb) VOX_TRY(st_sps(c));
+    VOX_TRY(st_vfraction(c));
+    VOX_TRY(st_viscosity(c));
+    hipEvent_t e0, e1;
+    VOX_HIP(hipEventCreate(&e0));
+    VOX_HIP(hipEventCreate(&e1));
+    int rc = VOX_OK;
+    if (c->p.precision == VOX_PREC_FP64) {
+        float best = 0.f;
+        int pick = 0;
+        for (int v = 0; v < kPredVariants && rc == VOX_OK; ++v) {
+            c->tune_pred = v;
+            if ((rc = st_prediction(c)) != VOX_OK) break;                // warm
+            (void)hipEventRecord(e0, c->stream);
+            for (int k = 0; k < reps && rc == VOX_OK; ++k) rc = st_prediction(c);
+            (void)hipEventRecord(e1, c->stream);
+            if (rc != VOX_OK) break;
+            if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(VOX_ERR_HIP, "autotune sync"); break; }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (v == 0 || ms < best) { best = ms; pick = v; }
+        }
+        c->tune_pred = pick;
+    }
+    if (rc == VOX_OK) rc = st_prediction(c);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    VOX_TRY(rc);
+    VOX_TRY(st_pnd(c, 1, 1));
+    if (wc) return VOX_OK;
+    VOX_TRY(st_bcon(c));
+    VOX_TRY(ppe_assemble(c));
+    if (c->nu <= 0) return VOX_OK;
+    float ms[kSpmvVariants] = {};
+    VOX_TRY(time_spmv_variants(c, reps, ms));
+    int pick = 0;
+    for (int v = 1; v < kSpmvVariants; ++v)
+        if (ms[v] < ms[pick]) pick = v;
+    c->tune_spmv = pick;
+    return VOX_OK;
+}
+
+}  // namespace vox
+
 extern "C" {
 
 const char* vox_last_error(void) { return g_err.c_str(); }
@@ -696,7 +753,33 @@ int vox_set_regions(vox_ctx* c, int nregion, const int32_t* region, const double
 int vox_autotune(vox_ctx* c, int reps, int32_t* modes)
 {
     VOX_TRY(need_init(c));
-    return autotune_lists(c, reps, modes);
+    VOX_TRY(autotune_lists(c, reps, modes));
+    return autotune_kernels(c, reps);
+}
+
+int vox_get_tuning(vox_ctx* c, int32_t* choice)
+{
+    VOX_TRY(check_ctx(c));
+    if (!choice) return fail(VOX_ERR_ARG, "null tuning array");
+    choice[0] = c->tuned_mode[0];
+    choice[1] = c->tuned_mode[1];
+    choice[2] = c->tune_pred;
+    choice[3] = c->tune_spmv;
+    return VOX_OK;
+}
+
+int vox_set_tuning(vox_ctx* c, const int32_t* choice)
+{
+    VOX_TRY(check_ctx(c));
+    if (!choice) return fail(VOX_ERR_ARG, "null tuning array");
+    if (choice[0] < 0 || choice[0] > 2 || choice[1] < 0 || choice[1] > 2 || choice[2] < 0 ||
+        choice[2] >= kPredVariants || choice[3] < 0 || choice[3] >= kSpmvVariants)
+        return fail(VOX_ERR_ARG, "tuning choice out of range");
+    c->tuned_mode[0] = choice[0];
+    c->tuned_mode[1] = choice[1];
+    c->tune_pred = choice[2];
+    c->tune_spmv = choice[3];
+    return VOX_OK;
 }
 
 int vox_set_n0(vox_ctx* c, double n0s, double n0l)

@@ -134,8 +134,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
 // matrix streams from HBM once per product (non-temporal loads keep it from
 // evicting the gathered vector from L2).
 
-// q = A p and the partial p.q of each block; one thread per row.
-template <class VT, class MT>
+// q = A p and the partial p.q of each block; one thread per row. U: entries
+// whose loads are issued together (the sum is in row order for every U, so
+// the autotuner's choice does not change any bit).
+template <class VT, class MT, int U = 8>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
@@ -152,7 +154,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
         const int len = a.rowlen[r];
         const int32_t* __restrict__ col = a.acol + mx_index(r, 0, a.cap);
         const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
-        constexpr int U = 8;
         int k = 0;
         for (; k + U <= len; k += U) {
             int32_t cc[U];
@@ -504,6 +505,17 @@ __global__ void k_scatter_x(int nu, const int32_t* __restrict__ urow, const doub
 }
 
 // ---- host ------------------------------------------------------------------------------------
+// The product variant the autotuner chose (8 entries per batch unless tuned).
+template <class VT, class MT>
+static void launch_spmv(vox_ctx* c, const CgArgs<VT, MT>& a, int variant)
+{
+    switch (variant) {
+    case 1: k_cg_spmv<VT, MT, 4><<<a.npq, kBlock, 0, c->stream>>>(a); break;
+    case 2: k_cg_spmv<VT, MT, 16><<<a.npq, kBlock, 0, c->stream>>>(a); break;
+    default: k_cg_spmv<VT, MT, 8><<<a.npq, kBlock, 0, c->stream>>>(a); break;
+    }
+}
+
 // Runs CG on A x = rhs to ||r|| <= tol ||rhs||. Returns iterations; *conv, *brk.
 template <class VT, class MT>
 static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
@@ -531,7 +543,7 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
             a.it = ++it;
             {
                 ProfScope ps(c, KC_SPMV);
-                k_cg_spmv<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
+                launch_spmv<VT, MT>(c, a, c->tune_spmv);
             }
             {
                 ProfScope ps(c, KC_CG_UPDATE);
@@ -582,7 +594,7 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
             a.it = ++it;
             {
                 ProfScope ps(c, KC_SPMV);
-                k_cg_spmv<VT, MT><<<a.npq, kBlock, 0, c->stream>>>(a);
+                launch_spmv<VT, MT>(c, a, c->tune_spmv);
             }
             {
                 ProfScope ps(c, KC_CG_UPDATE);
@@ -783,6 +795,54 @@ static int solve_learned(vox_ctx* c, vox_solve_info* info)
     if (rc == VOX_OK) c->lpc_baseline = c->st.last_iters > 0 ? c->st.last_iters : 1;
     c->st.lpc_baseline = c->lpc_baseline;
     return rc;
+}
+
+// Times each product variant on the assembled system (the context's solver
+// vectors hold p = D^-1 b afterwards, as after a solve's setup): ms[v] is the
+// mean of `reps` launches. Used by the autotuner; needs nu > 0.
+template <class VT, class MT>
+static int time_spmv_t(vox_ctx* c, CgArgs<VT, MT>& a, int reps, float* ms)
+{
+    k_cg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+    k_cg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
+    VOX_HIP(hipGetLastError());
+    hipEvent_t e0, e1;
+    VOX_HIP(hipEventCreate(&e0));
+    VOX_HIP(hipEventCreate(&e1));
+    int rc = VOX_OK;
+    for (int v = 0; v < kSpmvVariants && rc == VOX_OK; ++v) {
+        a.it = 1;
+        launch_spmv<VT, MT>(c, a, v);                          // warm
+        (void)hipEventRecord(e0, c->stream);
+        for (int k = 0; k < reps; ++k) launch_spmv<VT, MT>(c, a, v);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            rc = fail(VOX_ERR_HIP, "product timing failed");
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        ms[v] = t / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int time_spmv_variants(vox_ctx* c, int reps, float* ms)
+{
+    if (c->nu <= 0) return fail(VOX_ERR_STATE, "no assembled PPE to time the product on");
+    if (grid_for(c->nu) > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
+    if (c->p.precision == VOX_PREC_MIXED) {
+        if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
+        CgArgs<float, float> a;
+        fill_args(c, a, (const float*)c->aval32, c->cg_x32, c->cg_r32, c->cg_p32, c->cg_q32,
+                  c->rhs, c->p.cg_tol, c->p.cg_maxit);
+        return time_spmv_t(c, a, reps, ms);
+    }
+    if (!c->aval64_ok) VOX_TRY(ppe_make_aval64(c));
+    CgArgs<double, double> a;
+    fill_args(c, a, (const double*)c->aval, c->cg_x, c->cg_r, c->cg_p, c->cg_q, c->rhs,
+              c->p.cg_tol, c->p.cg_maxit);
+    return time_spmv_t(c, a, reps, ms);
 }
 
 int ppe_solve(vox_ctx* c, vox_solve_info* info)

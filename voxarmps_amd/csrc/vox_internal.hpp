@@ -18,6 +18,8 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
 constexpr int kPartialSlots = 10;    // scalars reduced through ctx->partials
+constexpr int kPredVariants = 5;     // predictor: records per group x block size
+constexpr int kSpmvVariants = 3;     // PPE product: entries per load batch (8, 4, 16)
 
 
 // Gather records: everything a neighbour loop reads about particle j in one
@@ -195,6 +197,8 @@ struct vox_ctx {
 
     vox::DevList list[2];
     int tuned_mode[2] = {0, 0};    // neighbour walk chosen by the autotuner (0: none)
+    int tune_pred = 0;             // predictor variant (vox::kPredVariants)
+    int tune_spmv = 0;             // PPE product variant (vox::kSpmvVariants)
     bool two_lists = false;
     double n0s = 0.0, n0l = 0.0;
     bool initialised = false;
@@ -370,6 +374,9 @@ int st_sps(vox_ctx* c);
 int st_vfraction(vox_ctx* c);
 int st_viscosity(vox_ctx* c);
 int st_prediction(vox_ctx* c);
+const char* pred_variant_name(int v);
+int autotune_kernels(vox_ctx* c, int reps);   // predictor and PPE product variants (vox_api.hip)
+int time_spmv_variants(vox_ctx* c, int reps, float* ms);   // vox_cg.hip
 bool uniform_meu(const vox_ctx* c);
 int st_prediction32(vox_ctx* c);
 int st_phat32(vox_ctx* c);          // the FP32 twins of the mixed precision mode

@@ -429,8 +429,12 @@ __global__ void k_pack_pred(PredArgs a)
 // 2 m m / (m + m) rounds back to m, so the harmonic-mean branch of
 // functions.cu:2328-2337 gives the same NEU as the same-type branch: the
 // neighbour's type is not read (one gathered line less per neighbour).
-template <bool kUniform>
-__global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
+// R: records gathered together per group (8 ids are loaded per row chunk and
+// consumed in groups of R; the terms are added in row order either way, so
+// every R gives the same bits), BS: threads per block. The autotuner picks
+// (R, BS) by timing.
+template <bool kUniform, int R, int BS>
+__global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
 {
     int i = xcd_tid();
     if (i >= a.n) return;
@@ -450,19 +454,19 @@ __global__ __launch_bounds__(kBlock) void k_prediction(PredArgs a)
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
     const int cnt = a.L.cnt[i];
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
-    // One row chunk (8 ids) per outer step, consumed in halves of 4: a half's
+    // One row chunk (8 ids) per outer step, consumed in groups of R: a group's
     // record and type loads are issued together, then its terms are
     // accumulated in row order.
     for (int g = 0; (g << 3) < cnt; ++g) {
     int32_t jc[8];
     load_chunk8(row, g, jc);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-    constexpr int U = 4;
+    for (int h = 0; h < 8 / R; ++h) {
+    constexpr int U = R;
     int jj[U], tj[U];
     PredRec rv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + 4 * h + u < cnt) ? jc[4 * h + u] : i;
+    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + U * h + u < cnt) ? jc[U * h + u] : i;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         rv[u] = a.rec[jj[u]];
@@ -529,6 +533,39 @@ bool uniform_meu(const vox_ctx* c)
     return hm == m;
 }
 
+// Predictor variants for the autotuner: (records per group, block size).
+static const int kPredR[kPredVariants] = {4, 8, 2, 4, 8};
+static const int kPredBS[kPredVariants] = {256, 256, 256, 128, 128};
+
+template <bool kU>
+static void launch_pred_u(const PredArgs& a, int v, int n, hipStream_t s)
+{
+    switch (v) {
+    case 1: k_prediction<kU, 8, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
+    case 2: k_prediction<kU, 2, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
+    case 3: k_prediction<kU, 4, 128><<<grid_for(n, 128), 128, 0, s>>>(a); break;
+    case 4: k_prediction<kU, 8, 128><<<grid_for(n, 128), 128, 0, s>>>(a); break;
+    default: k_prediction<kU, 4, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
+    }
+}
+
+static int launch_prediction(vox_ctx* c, const PredArgs& a, int variant)
+{
+    if (uniform_meu(c)) launch_pred_u<true>(a, variant, c->tp, c->stream);
+    else launch_pred_u<false>(a, variant, c->tp, c->stream);
+    VOX_HIP(hipGetLastError());
+    return VOX_OK;
+}
+
+const char* pred_variant_name(int v)
+{
+    static char buf[kPredVariants][48];
+    if (v < 0 || v >= kPredVariants) return "?";
+    std::snprintf(buf[v], sizeof buf[v], "%d records per group, %d threads per block", kPredR[v],
+                  kPredBS[v]);
+    return buf[v];
+}
+
 int st_prediction(vox_ctx* c)
 {
     if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
@@ -552,9 +589,7 @@ int st_prediction(vox_ctx* c)
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = c->pred_rec;
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    if (uniform_meu(c)) k_prediction<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    else k_prediction<false><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    VOX_HIP(hipGetLastError());
+    VOX_TRY(launch_prediction(c, a, c->tune_pred));
     return VOX_OK;
 }
 

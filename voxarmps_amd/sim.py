@@ -238,11 +238,22 @@ class Simulation:
                                                 _ptr(boxes, C.c_double)), "set_regions")
 
     def autotune(self, reps=3):
-        """Times both neighbour-search walks and keeps the faster per list;
-        returns the chosen walk (1 per particle, 2 per cell) for lists S, L."""
+        """Times both neighbour-search walks and keeps the faster per list,
+        then the predictor and PPE product variants; returns the chosen walk
+        (1 per particle, 2 per cell) for lists S, L (tuning() has all)."""
         m = np.zeros(2, np.int32)
         self.be.check(self.be.fn("autotune")(self.ctx, int(reps), _ptr(m, C.c_int32)), "autotune")
         return int(m[0]), int(m[1])
+
+    def tuning(self):
+        """(walk S, walk L, predictor variant, product variant)."""
+        t = np.zeros(4, np.int32)
+        self.be.check(self.be.fn("get_tuning")(self.ctx, _ptr(t, C.c_int32)), "get_tuning")
+        return tuple(int(v) for v in t)
+
+    def set_tuning(self, choice):
+        t = np.ascontiguousarray(choice, dtype=np.int32)
+        self.be.check(self.be.fn("set_tuning")(self.ctx, _ptr(t, C.c_int32)), "set_tuning")
 
     def attach_lpc(self, lpc):
         """Use a LearnedPreconditioner when the solver is the learned FCG
