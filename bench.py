@@ -506,7 +506,7 @@ def main():
     bytes_launch = spmv_bytes(nu, nnz, 4 if mixed else 8)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     per_kernel = {}
-    for k in ("list", "list_l", "cell", "sort", "pnd", "pred", "assemble", "spmv", "cg_update", "cg_p",
+    for k in ("list", "list_l", "cell", "sort", "pnd", "sps", "pred", "assemble", "spmv", "cg_update", "cg_p",
               "ir_residual", "lpc_net", "lpc_apply",
               "collision", "pgrad", "phat", "prep", "euler", "bcon", "visc", "vfrac", "misc"):
         t = C.c_double(0.0)
