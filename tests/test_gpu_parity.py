@@ -155,6 +155,53 @@ def test_prediction_single_fluid_bit_exact(hip, oracle, name, case, p):
     c.close()
 
 
+def test_compact_list_l_3d(hip, oracle):
+    """3D list L feeds PREDICTION, PNUM on list L (n0) and MATRIX with only the
+    entries of nonzero weight (d < re_l, the particle itself left out); the
+    reference rows hold d <= re_l + DELTA and the particle itself. Consumers on
+    the compact rows, never exported first, are bit-exact; an export then
+    gives the reference rows; and after EULER moves the particles, PREDICTION
+    and MATRIX on the list of the earlier positions (no NEIGHBOR in between)
+    read the reference rows, as the reference does, and are bit-exact too."""
+    case = vx.jitter(vx.dam_break(3, 0.005, (16, 16, 12), (48, 20, 13)), 0.1 * 0.005, seed=5)
+    p = vx.dam3d_params(case, tol=1e-10)
+    g, c = _pair(case, p, hip, oracle)
+    g.init()
+    c.init()
+    assert g.n0() == c.n0(), "n0 on the compact list L"
+    for sim in (g, c):
+        random_state(sim, seed=13)
+    _sync_fields(c, g, ["U", "V", "W", "UNEW", "VNEW", "WNEW", "PNEW", "PHAT", "P"])
+    for sim in (g, c):
+        sim.NEIGHBOR()
+    for st in ("PNUM", "VISCOSITY", "PREDICTION", "PNUMSTAR"):
+        getattr(g, st)()
+        getattr(c, st)()
+        _assert_same(g, c, FLOAT_OUT[st], f"compact {st}")
+    for sim in (g, c):
+        sim.BCON()
+        sim.MATRIX()
+    _assert_same(g, c, ["SOURCE"], "compact MATRIX source")
+    Ag, Ac = g.ppe(), c.ppe()
+    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+        assert np.array_equal(Ag[k], Ac[k]), f"compact MATRIX {k}"
+    _assert_lists(g, c, "export after compact consumers")
+    # a fresh compact list, then motion without a new search
+    for sim in (g, c):
+        sim.NEIGHBOR()
+        sim.PREDICTION()
+        sim.set("UNEW", sim.get("U") + 0.5)
+        sim.EULERINTEGRATION()
+        sim.PREDICTION()
+        sim.MATRIX()
+    _assert_same(g, c, FLOAT_OUT["PREDICTION"], "stale list PREDICTION")
+    Ag, Ac = g.ppe(), c.ppe()
+    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+        assert np.array_equal(Ag[k], Ac[k]), f"stale list MATRIX {k}"
+    g.close()
+    c.close()
+
+
 @pytest.mark.parametrize("name,case,p", list(_stage_cases()), ids=lambda v: v if isinstance(v, str) else "")
 def test_stages_bit_exact(hip, oracle, name, case, p):
     case = case.copy()

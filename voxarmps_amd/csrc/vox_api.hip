@@ -543,6 +543,7 @@ int vox_upload(vox_ctx* c, int field, const void* host)
     VOX_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, c->stream));
     VOX_HIP(hipStreamSynchronize(c->stream));
     if (field == VOX_F_MEU) c->meu_uniform = false;
+    if (field == VOX_F_X || field == VOX_F_Y || field == VOX_F_Z) ++c->pos_epoch;
     if (field == VOX_F_PTYPE) {          // types never change during a run
         const int32_t* t = static_cast<const int32_t*>(host);
         long n2 = 0;
@@ -794,6 +795,7 @@ int vox_get_neighbors(vox_ctx* c, int which, int32_t* counts, int32_t* ids, int 
 {
     VOX_TRY(check_ctx(c));
     if (which != VOX_LIST_S && which != VOX_LIST_L) return fail(VOX_ERR_ARG, "bad list id");
+    if (which == VOX_LIST_L && c->two_lists) VOX_TRY(ensure_reference_list(c, 1));
     const DevList& L = list_of(c, which);
     const size_t n = (size_t)c->tp;
     std::vector<int32_t> cnt(n);

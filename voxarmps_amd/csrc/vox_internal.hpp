@@ -103,6 +103,9 @@ struct DevList {
     int32_t* order_raw = nullptr;  // tp (atomic scatter result)
     PosRec* rec = nullptr;         // tp: positions + id sorted by (cell, id)
     long tnc_alloc = 0;
+    bool compact = false;          // rows hold only the entries of nonzero weight (see
+                                   // list_compact_ok); the reference rows are rebuilt on demand
+    uint64_t epoch = 0;            // vox_ctx::pos_epoch when the rows were searched
 };
 
 // Sliced row storage in chunks of 8 (SELL-64x8). Rows are grouped in slices
@@ -191,6 +194,8 @@ struct vox_ctx {
     double* f[VOX_F_COUNT] = {};
     int32_t* ptype = nullptr;
     long n_fluid2 = -1;            // PTYPE 2 particles in the uploaded types (-1: not known)
+    uint64_t pos_epoch = 0;        // advanced by every write of x, y, z (EULER, COLLISION2,
+                                   // PREPDATA, uploads)
     bool meu_uniform = false;      // MEU was last written by the Newtonian VISCOSITY with no
                                    // PTYPE 2 particle: nu1 rho1 everywhere
     int32_t* bcon = nullptr;
@@ -365,6 +370,9 @@ int read_flags(vox_ctx* c);   // D2H of dflags + dscal into pinned mirrors (sync
 
 int autotune_lists(vox_ctx* c, int reps, int32_t* modes);
 int build_neighbors(vox_ctx* c);
+bool list_compact_ok(const vox_ctx* c);
+int ensure_reference_list(vox_ctx* c, int which);
+int list_l_for_consumer(vox_ctx* c);
 int ensure_list_storage(vox_ctx* c, int which, int cap);
 
 // stages (vox_stages.hip)

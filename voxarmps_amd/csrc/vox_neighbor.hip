@@ -531,6 +531,28 @@ int ensure_list_storage(vox_ctx* c, int which, int cap)
 
 static int search_one(vox_ctx* c, int which, double re);
 
+// List L in 3D can hold compact rows: the reference accepts d <= re + DELTA
+// with the particle itself (functions.cu:2886), but its consumers, PREDICTION
+// and MATRIX, weigh each entry by W(d) with W = 0 at d >= re (kernel types 2
+// and 6) and skip the particle itself, so the entries they actually use are
+// those at d < re other than the particle itself - the 2D accept rule at the
+// same radius. The compact rows keep the reference order (a subsequence of it,
+// about 18% shorter), and a sum over them adds exactly the nonzero terms of
+// the reference's sum in the same order (the skipped terms are +0 added to a
+// sum that is never -0), so every result is the same bits for finite fields
+// (a non-finite neighbour value times W = 0 is NaN in the reference's sum).
+// MATRIX on the CPU path assembles at the positions the list was searched at
+// (x, not x*), so its distances are the search's own. matopt and srcopt weigh
+// by re/d^3 at any d and need the reference rows, as does an export:
+// ensure_reference_list rebuilds them from the cell-sorted records of the same
+// NEIGHBOR call.
+bool list_compact_ok(const vox_ctx* c)
+{
+    const vox_params& p = c->p;
+    return c->two_lists && p.dim == 3 && !p.matopt && !p.srcopt && !p.ref_gpu &&
+           (p.ktype == 2 || p.ktype == 6);
+}
+
 // Expected particles per cell (cell volume / DL^dim) below 24: the cells are
 // sorted one thread per particle rather than one wave per cell.
 static bool few_per_cell(const vox_ctx* c, const DevList& L)
@@ -598,7 +620,13 @@ static int search_one(vox_ctx* c, int which, double re)
     const int n = c->tp;
     const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
     const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
-    double R = L.g.incl ? re + c->p.delta : re;
+    // compact rows (list L): d < re, the particle itself excluded
+    const bool compact = (which == 1) && list_compact_ok(c);
+    Grid gg = L.g;
+    if (compact) gg.incl = 0;
+    const double R = gg.incl ? re + c->p.delta : re;
+    L.compact = compact;
+    L.epoch = c->pos_epoch;
     // Walk per cell when cells hold enough particles to fill most of a wave
     // (expected count = cell volume / DL^dim), else per particle.
     const double per_cell = (c->p.dim == 3)
@@ -612,11 +640,11 @@ static int search_one(vox_ctx* c, int which, double re)
     if (cell_walk) {
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
         k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
-            L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+            gg, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
     } else {
         const int nb = grid_for(n);
         k_build_list<<<nb, kBlock, 0, c->stream>>>(
-            L.g, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
+            gg, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
             L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
         k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
     }
@@ -660,6 +688,48 @@ static int regrow_one(vox_ctx* c, int which, double re)
         VOX_TRY(read_flags(c));
     }
     return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+// The reference rows of list `which` in place of compact ones, searched again
+// over the cell-sorted records of the NEIGHBOR call that built the list (the
+// positions of that call, whatever the particles did since), one wave per
+// cell, with wider rows while any overflows.
+int ensure_reference_list(vox_ctx* c, int which)
+{
+    DevList& L = c->list[which];
+    if (!L.compact) return VOX_OK;
+    const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
+    const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
+    const double R = L.g.incl ? L.re + c->p.delta : L.re;
+    if (L.g.tnc * kCellHalves >= (1L << 31)) return fail(VOX_ERR_ARG, "neighbour grid too large");
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
+        VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
+        k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
+            L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+        VOX_HIP(hipGetLastError());
+        VOX_TRY(read_flags(c));
+        if (c->hflags[fo] == 0) {
+            L.compact = false;
+            return VOX_OK;
+        }
+        const int need = c->hflags[fm];
+        VOX_TRY(ensure_list_storage(c, which, ((need + need / 4) + 7) & ~7));
+    }
+    return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+// Called by every consumer of list L before it reads the rows: compact rows
+// that no longer stand for the reference rows - the particles have moved since
+// the search (the reference's rows, searched with the DELTA margin, are then
+// read at other positions), or the settings have changed (a weight function
+// that is not zero at re, matopt, srcopt) - are replaced by the reference rows.
+int list_l_for_consumer(vox_ctx* c)
+{
+    const DevList& L = c->list[1];
+    if (c->two_lists && L.compact && (L.epoch != c->pos_epoch || !list_compact_ok(c)))
+        return ensure_reference_list(c, 1);
+    return VOX_OK;
 }
 
 static int build_one(vox_ctx* c, int which, double re)

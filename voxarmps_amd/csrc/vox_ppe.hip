@@ -346,6 +346,7 @@ __global__ void k_rhs_from_source(int nu, const int32_t* __restrict__ urow, cons
 
 int ppe_assemble(vox_ctx* c)
 {
+    VOX_TRY(list_l_for_consumer(c));
     ProfScope ps(c, KC_ASSEMBLE);
     const int n = c->tp;
     const DevList& L = list_of(c, VOX_LIST_L);

@@ -126,6 +126,7 @@ int st_n0(vox_ctx* c)
     for (double v : h) if (v > mx) mx = v;
     c->n0s = mx;
     if (c->two_lists) {
+        VOX_TRY(list_l_for_consumer(c));
         launch_pnd(c, list_of(c, VOX_LIST_L), c->p.re_l, c->p.ncorrection, c->f[VOX_F_X],
                    c->f[VOX_F_Y], c->f[VOX_F_Z], c->scratch_d, nullptr, nullptr);
         VOX_HIP(hipGetLastError());
@@ -568,6 +569,7 @@ const char* pred_variant_name(int v)
 
 int st_prediction(vox_ctx* c)
 {
+    VOX_TRY(list_l_for_consumer(c));
     if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
     ProfScope ps(c, KC_PRED);
     const vox_params& p = c->p;
@@ -844,6 +846,7 @@ __global__ void k_euler(EulerArgs a)
 int st_euler(vox_ctx* c)
 {
     ProfScope ps(c, KC_EULER);
+    ++c->pos_epoch;
     const vox_params& p = c->p;
     EulerArgs a;
     a.first = c->gp + c->wp; a.n = c->tp; a.dim = p.dim;
@@ -1224,6 +1227,7 @@ constexpr double kCollMarginBase = 0.1;     // screen margin, fraction of coll
 int st_collision(vox_ctx* c)
 {
     ProfScope ps(c, KC_COLL);
+    ++c->pos_epoch;
     const vox_params& p = c->p;
     const int n = c->tp;
     double* X = c->f[VOX_F_X]; double* Y = c->f[VOX_F_Y]; double* Z = c->f[VOX_F_Z];
@@ -1371,6 +1375,7 @@ __global__ __launch_bounds__(kBlock) void k_prepdata(PrepArgs a)
 int st_prepdata(vox_ctx* c)
 {
     ProfScope ps(c, KC_PREP);
+    ++c->pos_epoch;
     const vox_params& p = c->p;
     PrepArgs a;
     a.n = c->tp; a.dim = p.dim;
