@@ -221,8 +221,6 @@ struct vox_ctx {
     int32_t* rowlen = nullptr;     // tp: stored off-diagonal entries of each row
     int32_t* acol = nullptr;       // cap_l * tp, mx_index
     double* aval = nullptr;        // cap_l * tp, mx_index
-    int32_t* tcol = nullptr;       // cap_l * tp, sl_index: the assembly's staging rows
-    double* tval = nullptr;
     double* adiag = nullptr;       // tp
     double* rhs = nullptr;         // tp
     int a_cap = 0;

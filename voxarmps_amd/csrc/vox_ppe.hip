@@ -42,190 +42,128 @@ struct AsmArgs {
     double inv_re, inv_n0l, inv_rho1;     // correctly rounded reciprocals
     ListView L;
     const double *x, *y, *z, *pnd;
-    const int32_t *bcon, *uidx;
-    int32_t *acol, *rowlen, *tcol;         // tcol/tval: chunked staging rows
-    double *aval, *adiag, *rhs, *source, *tval;
+    const int32_t *bcon, *uidx, *urow;
+    const int32_t* nu;                     // device: the number of unknowns
+    int32_t *acol, *rowlen;
+    double *aval, *adiag, *rhs, *source;
+    float* aval32;                         // FP32 values (mixed precision, learned network)
     PosRec* rec;
 };
 
-// 32-byte gather record per particle: position, bcon (id slot), unknown index.
+// 32-byte gather record per particle: position, bcon (id slot), unknown
+// index; and the source term of the default MATRIX (functions.cu:990-1001),
+// 0 for every particle that is not an unknown.
 __global__ void k_pack_asm(AsmArgs a)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     PosRec r;
     r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
-    r.id = a.bcon[i];
+    const int bi = a.bcon[i];
+    r.id = bi;
     r.aux = a.uidx[i];
     a.rec[i] = r;
+    a.source[i] = (bi == 0) ? a.inv_dt_dt * (a.pnd[i] - a.n0s) / a.n0s : 0.0;
 }
 
 // Per-thread ring of the last 16 outputs in LDS (strides of 17 keep the lanes'
 // equal-slot accesses on distinct banks).
 constexpr int kRing = 17;
 
-// Stores chunk c (8 entries) of the ring to the chunked staging rows: one
-// 32-byte and one 64-byte contiguous store per lane.
-__device__ __forceinline__ void flush_chunk(const int32_t* rc, const double* rv, int c,
-                                            int32_t* __restrict__ tcol, double* __restrict__ tval,
-                                            long rb)
-{
-    const int s0 = (c & 1) * 8;
-    const long o = rb + ((long)c << 9);
-    v4i* c4 = reinterpret_cast<v4i*>(tcol + o);
-    c4[0] = v4i{rc[s0 + 0], rc[s0 + 1], rc[s0 + 2], rc[s0 + 3]};
-    c4[1] = v4i{rc[s0 + 4], rc[s0 + 5], rc[s0 + 6], rc[s0 + 7]};
-    v2d* v2 = reinterpret_cast<v2d*>(tval + o);
-    v2[0] = v2d{rv[s0 + 0], rv[s0 + 1]};
-    v2[1] = v2d{rv[s0 + 2], rv[s0 + 3]};
-    v2[2] = v2d{rv[s0 + 4], rv[s0 + 5]};
-    v2[3] = v2d{rv[s0 + 6], rv[s0 + 7]};
-}
-// Mixed precision: the staging values are the FP64 values rounded to FP32
-// (one 32-byte store per chunk).
-__device__ __forceinline__ void flush_chunk(const int32_t* rc, const double* rv, int c,
-                                            int32_t* __restrict__ tcol, float* __restrict__ tval,
-                                            long rb)
-{
-    const int s0 = (c & 1) * 8;
-    const long o = rb + ((long)c << 9);
-    v4i* c4 = reinterpret_cast<v4i*>(tcol + o);
-    c4[0] = v4i{rc[s0 + 0], rc[s0 + 1], rc[s0 + 2], rc[s0 + 3]};
-    c4[1] = v4i{rc[s0 + 4], rc[s0 + 5], rc[s0 + 6], rc[s0 + 7]};
-    v4f* f4 = reinterpret_cast<v4f*>(tval + o);
-    f4[0] = v4f{(float)rv[s0 + 0], (float)rv[s0 + 1], (float)rv[s0 + 2], (float)rv[s0 + 3]};
-    f4[1] = v4f{(float)rv[s0 + 4], (float)rv[s0 + 5], (float)rv[s0 + 6], (float)rv[s0 + 7]};
-}
-
-// SV: staging value type, double, or float for mixed precision (the values
-// are computed in FP64 either way; the diagonal sums them in FP64).
+// One thread per unknown row r (particle urow[r]), so a wave is one SELL-64
+// slice: its lanes produce their rows' entries 8 at a time into LDS, and once
+// every lane holds its group g (or has run out of row), the wave stores entry
+// 8g + u of its 64 rows as one contiguous block for u = 0..7. Rows are written
+// in their final layout in one pass. Entries are processed strictly in list
+// order (the reference's), so the values and the diagonal are bit-identical
+// to the sequential MATRIX.
+// SV = double: FP64 values (and their FP32 copy when aval32 is set, for the
+// learned preconditioner's network); SV = float: mixed precision, FP32 values
+// only (computed in FP64 and rounded once; the diagonal stays FP64).
 template <class SV>
-__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
+__global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
 {
-    SV* __restrict__ tval = reinterpret_cast<SV*>(a.tval);
     __shared__ int32_t ring_c[kBlock * kRing];
     __shared__ double ring_v[kBlock * kRing];
-    int i = xcd_tid();
-    if (i >= a.n) return;
-    const int bi = a.bcon[i];
-    // right-hand side of the default source term (functions.cu:990-1001)
-    double src = 0.0;
-    if (bi == 0) src = a.inv_dt_dt * (a.pnd[i] - a.n0s) / a.n0s;
-    a.source[i] = src;
-    if (bi != 0) return;
-    const int r = a.uidx[i];
-    const long rb = sl_index(r, 0, a.acap);               // chunked staging rows
+    const int nu = *a.nu;
+    if ((long)blockIdx.x * blockDim.x >= nu) return;          // uniform over the block
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = r < nu;
+    const int i = live ? a.urow[r] : 0;
     int32_t* rc = ring_c + threadIdx.x * kRing;
     double* rv = ring_v + threadIdx.x * kRing;
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
     const PosRec me = a.rec[i];
     const double xi = me.x, yi = me.y, zi = me.z;
+    const long ob = mx_index(r, 0, a.acap);                  // entry k at ob + 64 k
     double diag = 0.0;
-    int k = 0;
-    const int cnt = a.L.cnt[i];
-    // One row chunk of 8 per step: all index and record loads of a chunk are
-    // issued before its stores; entries are processed strictly in row order.
-    // Kept entries go to the thread's LDS ring; each completed group of 8 is
-    // stored at once (whole sectors), so no partly written line is left to be
-    // written back again and again.
+    int k = 0, l0 = 0;
+    const int cnt = live ? a.L.cnt[i] : 0;
     constexpr int U = 8;
-    for (int l0 = 0; l0 < cnt; l0 += U) {
-        int jj[U];
-        PosRec rr[U];
-        load_chunk8(row, l0 >> 3, jj);
+    for (int g = 0;; ++g) {
+        // this lane's rows until group g is complete or the row is done; a
+        // list chunk adds at most 8 entries, so the ring never holds more
+        // than groups g and g + 1
+        while (k < 8 * (g + 1) && l0 < cnt) {
+            int jj[U];
+            PosRec rr[U];
+            load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+            for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
 #pragma unroll
-        for (int u = 0; u < U; ++u) rr[u] = a.rec[jj[u]];
-        const int k0 = k;
-        // Branch-free per entry (a skipped entry keeps diag unchanged through a
-        // select), so no record load is sunk into a branch behind a full wait.
+            for (int u = 0; u < U; ++u) rr[u] = a.rec[jj[u]];
+            // Branch-free per entry (a skipped entry keeps diag unchanged
+            // through a select), so no record load is sunk into a branch
+            // behind a full wait.
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const PosRec rj = rr[u];
-            const int bj = rj.id;
-            // z is 0 in 2D records, and adding 0*0 leaves the 2D sum unchanged,
-            // so one expression serves both dimensions without a branch
-            const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            // jj == i: self, or padding past the row end
-            const bool use = (jj[u] != i) && (bj != -1) && !(fabs(d) <= 1.0e-15);
-            // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by);
-            // matopt: hl / d^3 / n0l / rho1 (functions.cu:903-913)
-            const double val = a.matopt
-                ? div_by(div_by(a.hl / (d * d * d), a.n0l, a.inv_n0l), a.rho1, a.inv_rho1)
-                : div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
-                         a.rho1, a.inv_rho1);
-            if (use && bj == 0 && val != 0.0) {
-                rc[k & 15] = rj.aux;
-                rv[k & 15] = -val;
-                ++k;
+            for (int u = 0; u < U; ++u) {
+                const PosRec rj = rr[u];
+                const int bj = rj.id;
+                // z is 0 in 2D records, and adding 0*0 leaves the 2D sum
+                // unchanged, so one expression serves both dimensions
+                const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
+                const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                // jj == i: self, or padding past the row end
+                const bool use = (jj[u] != i) && (bj != -1) && !(fabs(d) <= 1.0e-15);
+                // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by);
+                // matopt: hl / d^3 / n0l / rho1 (functions.cu:903-913)
+                const double val = a.matopt
+                    ? div_by(div_by(a.hl / (d * d * d), a.n0l, a.inv_n0l), a.rho1, a.inv_rho1)
+                    : div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
+                             a.rho1, a.inv_rho1);
+                if (use && bj == 0 && val != 0.0) {
+                    rc[k & 15] = rj.aux;
+                    rv[k & 15] = -val;
+                    ++k;
+                }
+                diag = use ? diag + val : diag;
             }
-            diag = use ? diag + val : diag;
+            l0 += U;
         }
-        // at most 8 outputs per step: at most one group of 8 completed
-        if ((k0 >> 3) != (k >> 3)) flush_chunk(rc, rv, k0 >> 3, a.tcol, tval, rb);
-    }
-    if (k & 7) flush_chunk(rc, rv, k >> 3, a.tcol, tval, rb);   // the last, partial group
-    diag += a.dshift;
-    a.adiag[r] = diag;
-    a.rowlen[r] = k;
-    a.rhs[r] = src;
-}
-
-// The packed rows from the chunked staging layout into SELL-64: a slice's
-// group g of 64 rows is one contiguous block there (lane t's 8 entries at
-// offset 8t), and entry 8g + u of the 64 rows is one contiguous block here, so
-// every load and store of a wave is contiguous. One thread per row.
-// SV = double: FP64 staging to aval (and aval32 when given, for the learned
-// preconditioner); SV = float: FP32 staging to aval32 only (mixed precision).
-template <class SV>
-__global__ __launch_bounds__(kBlock) void k_asm_transpose(int nu, int cap,
-                                                          const int32_t* __restrict__ rowlen,
-                                                          const int32_t* __restrict__ tcol,
-                                                          const SV* __restrict__ tval,
-                                                          int32_t* __restrict__ acol,
-                                                          double* __restrict__ aval,
-                                                          float* __restrict__ aval32)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nu) return;
-    const long rb = sl_index(r, 0, cap);
-    const long ob = mx_index(r, 0, cap);
-    const int len = rowlen[r];
-    for (int g = 0; (g << 3) < len; ++g) {
-        int32_t cc[8];
-        SV vv[8];
-        {
-            const v4i* cp = reinterpret_cast<const v4i*>(tcol + rb + ((long)g << 9));
-            const v4i c0 = cp[0], c1 = cp[1];
-            cc[0] = c0.x; cc[1] = c0.y; cc[2] = c0.z; cc[3] = c0.w;
-            cc[4] = c1.x; cc[5] = c1.y; cc[6] = c1.z; cc[7] = c1.w;
-            if constexpr (sizeof(SV) == 8) {
-                const v2d* vp = reinterpret_cast<const v2d*>(tval + rb + ((long)g << 9));
+        const bool has = k > 8 * g;
+        if (!__any(has)) break;                                // every row is written
+        if (has) {
+            const int s0 = (g & 1) * 8;
 #pragma unroll
-                for (int h = 0; h < 4; ++h) { const v2d t = vp[h]; vv[2 * h] = t.x; vv[2 * h + 1] = t.y; }
-            } else {
-                const v4f* vp = reinterpret_cast<const v4f*>(tval + rb + ((long)g << 9));
-                const v4f f0 = vp[0], f1 = vp[1];
-                vv[0] = f0.x; vv[1] = f0.y; vv[2] = f0.z; vv[3] = f0.w;
-                vv[4] = f1.x; vv[5] = f1.y; vv[6] = f1.z; vv[7] = f1.w;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int kk = (g << 3) + u;
-            if (kk < len) {
-                acol[ob + 64L * kk] = cc[u];
-                if constexpr (sizeof(SV) == 8) {
-                    aval[ob + 64L * kk] = vv[u];
-                    if (aval32) aval32[ob + 64L * kk] = (float)vv[u];
-                } else {
-                    aval32[ob + 64L * kk] = vv[u];
+            for (int u = 0; u < 8; ++u) {
+                const int kk = 8 * g + u;
+                if (kk < k) {
+                    const long o = ob + 64L * kk;
+                    a.acol[o] = rc[s0 + u];
+                    if constexpr (sizeof(SV) == 8) {
+                        a.aval[o] = rv[s0 + u];
+                        if (a.aval32) a.aval32[o] = (float)rv[s0 + u];
+                    } else {
+                        a.aval32[o] = (float)rv[s0 + u];
+                    }
                 }
             }
         }
     }
+    if (!live) return;
+    a.adiag[r] = diag + a.dshift;
+    a.rowlen[r] = k;
+    a.rhs[r] = a.source[i];
 }
 
 // FP64 copy of FP32-stored values (a solve in FP64 of a system assembled in
@@ -351,12 +289,12 @@ int ppe_assemble(vox_ctx* c)
     const int n = c->tp;
     const DevList& L = list_of(c, VOX_LIST_L);
     if (c->a_cap < L.cap) {
-        for (int32_t** q : {&c->acol, &c->tcol}) { if (*q) (void)hipFree(*q); *q = nullptr; }
-        for (double** q : {&c->aval, &c->tval}) { if (*q) (void)hipFree(*q); *q = nullptr; }
+        if (c->acol) (void)hipFree(c->acol);
+        if (c->aval) (void)hipFree(c->aval);
+        c->acol = nullptr;
+        c->aval = nullptr;
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(n, L.cap)));
-        VOX_HIP(hipMalloc(&c->tcol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
-        VOX_HIP(hipMalloc(&c->tval, sizeof(double) * (size_t)sl_size(n, L.cap)));
         if (c->aval32) (void)hipFree(c->aval32);
         c->aval32 = nullptr;
         c->aval32_ok = false;
@@ -368,6 +306,8 @@ int ppe_assemble(vox_ctx* c)
     // preconditioner's network.
     const bool fp32_only = (c->p.precision == VOX_PREC_MIXED);
     const bool mixed = fp32_only || (c->p.solver == VOX_SOLVER_LEARNED_FCG && c->lpc);
+    if (mixed && !c->aval32)
+        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
     k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
     VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
     k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
@@ -389,33 +329,20 @@ int ppe_assemble(vox_ctx* c)
     const bool star = p.ref_gpu != 0;
     a.x = c->f[star ? VOX_F_XSTAR : VOX_F_X]; a.y = c->f[star ? VOX_F_YSTAR : VOX_F_Y];
     a.z = c->f[star ? VOX_F_ZSTAR : VOX_F_Z]; a.pnd = c->f[star ? VOX_F_NSTAR : VOX_F_N];
-    a.bcon = c->bcon; a.uidx = c->uidx;
+    a.bcon = c->bcon; a.uidx = c->uidx; a.urow = c->urow; a.nu = c->dflags + FLAG_NU;
     a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
-    a.tcol = c->tcol; a.tval = c->tval;
+    a.aval32 = mixed ? c->aval32 : nullptr;
     a.adiag = c->adiag; a.rhs = c->rhs;
     a.source = c->f[VOX_F_SOURCE];
     a.rec = c->pos_rec;
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
-    if (fp32_only) k_assemble<float><<<grid_for(n), kBlock, 0, c->stream>>>(a);
-    else k_assemble<double><<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    // rows: a grid for every particle, of which the blocks past nu (read on
+    // the device) return at once
+    if (fp32_only) k_assemble_rows<float><<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    else k_assemble_rows<double><<<grid_for(n), kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
-    c->aval32_ok = false;
-    c->aval64_ok = false;
-    if (mixed && !c->aval32)
-        VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
-    if (c->nu > 0) {
-        if (fp32_only)
-            k_asm_transpose<float><<<grid_for(c->nu), kBlock, 0, c->stream>>>(
-                c->nu, c->a_cap, c->rowlen, c->tcol, reinterpret_cast<const float*>(c->tval),
-                c->acol, c->aval, c->aval32);
-        else
-            k_asm_transpose<double><<<grid_for(c->nu), kBlock, 0, c->stream>>>(
-                c->nu, c->a_cap, c->rowlen, c->tcol, c->tval, c->acol, c->aval,
-                mixed ? c->aval32 : nullptr);
-        VOX_HIP(hipGetLastError());
-    }
     c->aval32_ok = mixed;
     c->aval64_ok = !fp32_only;
     if (p.srcopt) {
@@ -577,12 +504,12 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
     }
     const int cap = round_cap(std::max(maxoff, 1));
     if (c->a_cap < cap) {
-        for (int32_t** q : {&c->acol, &c->tcol}) { if (*q) (void)hipFree(*q); *q = nullptr; }
-        for (double** q : {&c->aval, &c->tval}) { if (*q) (void)hipFree(*q); *q = nullptr; }
+        if (c->acol) (void)hipFree(c->acol);
+        if (c->aval) (void)hipFree(c->aval);
+        c->acol = nullptr;
+        c->aval = nullptr;
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(c->tp, cap)));
-        VOX_HIP(hipMalloc(&c->tcol, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
-        VOX_HIP(hipMalloc(&c->tval, sizeof(double) * (size_t)sl_size(c->tp, cap)));
         if (c->aval32) (void)hipFree(c->aval32);
         c->aval32 = nullptr;
         c->a_cap = cap;
