@@ -120,7 +120,8 @@ typedef struct vox_params {
                              of the previous step (field P) and refines in FP64;
                              0: from zero, as the reference (functions.cu:2503) */
     int32_t list_mode;    /* neighbour search: 0 auto, 1 one thread per particle,
-                             2 one wave per grid cell (same lists either way)    */
+                             2 one wave per 64 cell-sorted particles (same
+                             lists either way)                                 */
     int32_t lpc_cap_pct;  /* learned FCG: iteration cap in percent of the Jacobi-CG
                              baseline before the fallback (200: twice)         */
     int32_t method;       /* PRESSURECALC: 1 PPE (incompressible), 3 weakly
@@ -275,8 +276,8 @@ int vox_get_status(vox_ctx* ctx, vox_status* st);
 int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const double* boxes);
 
 /* Tile-shape autotuner (no counterpart in the reference): times both
- * neighbour-search walks (1: one thread per particle, 2: one wave per grid
- * cell) over `reps` builds on the current positions and keeps the faster per
+ * neighbour-search walks (1: one thread per particle, 2: one wave per 64
+ * cell-sorted particles) over `reps` builds on the current positions and keeps the faster per
  * list (modes[2] receives the choice for lists S and L); then runs the step's
  * stages up to MATRIX once and times the predictor's variants (FP64: records
  * gathered per group 4, 8 or 2 with 256 threads per block, 4 or 8 with 128)

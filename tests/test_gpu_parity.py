@@ -79,7 +79,7 @@ def _cases():
 @pytest.mark.parametrize("mode", [0, 1, 2], ids=["auto", "per-particle", "per-cell"])
 @pytest.mark.parametrize("name,case,p", list(_cases()), ids=lambda v: v if isinstance(v, str) else "")
 def test_neighbor_lists_bit_exact(hip, oracle, name, case, p, mode):
-    """Both search kernels (one thread per particle, one wave per cell) give
+    """Both search kernels (one thread per particle, one wave per 64 sorted particles) give
     the oracle's lists."""
     p = p.copy()
     p.list_mode = mode

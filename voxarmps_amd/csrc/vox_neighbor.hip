@@ -21,7 +21,7 @@
 // Pipeline per list: key+count (atomic histogram) -> exclusive scan (cell
 // start) -> atomic scatter -> per-cell rank sort by id (one wave per cell,
 // LDS tile) with position records copied into cell order -> per-particle stencil
-// walk (one thread per particle, or one wave per cell) writing sliced rows
+// walk (one thread per particle, or a wave per 64 sorted particles) writing sliced rows
 // (sl_index). With two radii the two lists' pipelines run on two streams.
 #include <utility>
 
@@ -252,7 +252,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(int n, const int32_t* __rest
 }
 
 // One wave per cell: rank of each id among the cell's ids (ids are unique), so
-// the run becomes ascending by id; positions are copied into the same order.
+// the run becomes ascending by id; positions are copied into the same order,
+// with the cell key in the record's aux slot.
 constexpr int kSortTile = 512;   // ids per wave held in LDS
 
 __global__ __launch_bounds__(kBlock) void k_cell_sort(
@@ -285,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
         rr.y = y[id];
         rr.z = (dim == 3) ? z[id] : 0.0;
         rr.id = id;
-        rr.aux = 0;
+        rr.aux = (int32_t)cellid;
         rec[pos] = rr;
     }
 }
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(
     rr.y = y[id];
     rr.z = (dim == 3) ? z[id] : 0.0;
     rr.id = id;
-    rr.aux = 0;
+    rr.aux = c;
     rec[s + rank] = rr;
 }
 
@@ -415,52 +416,63 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
     (void)flag_max;
 }
 
-// Cell-tiled stencil walk for large cells (3D list L: 64-80 particles per
-// cell). Every particle of a cell has the same candidate sequence (the stencil
-// cells in reference order, ascending id inside each), so one wave takes up to
-// 64 particles of a cell, streams each chunk of 64 candidate records once into
-// LDS and every lane tests every candidate (broadcast reads): the records are
-// read once per wave instead of once per particle. Block b serves cell b / 2;
-// its half g = b % 2 takes the cell's particles 64 g + 128 k. The accept test,
-// the order and the row storage are those of k_build_list.
-constexpr int kCellHalves = 2;
-
-__global__ __launch_bounds__(kWave) void k_build_list_cells(
-    Grid g, double R, const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
+// Stencil walk for large cells (3D list L: 64-80 particles per cell). A wave
+// takes 64 consecutive particles of the cell-sorted order - one to a few
+// cells of one x-row of the grid - and streams the union of their stencils
+// once: x-blocks from the first lane's cell - 1 to the last lane's + 1, in
+// each the (y, z) cells of the row's stencil in reference order, each cell's
+// records in chunks of 64 through LDS, every lane testing every candidate
+// (broadcast reads). A lane takes the candidates of its own three x-blocks
+// only; its candidate sequence is therefore exactly its stencil in reference
+// order (x outer, then y, then z, ascending id inside a cell). Lanes of
+// another grid row, or more than one cell further along x than the group's
+// first lane, are served by the next pass of the same wave, so a pass never
+// streams more than four x-blocks. Against one wave per cell, lanes stay
+// filled where a cell holds more than 64 particles or fewer.
+// The accept test screens with a fused distance (within a few ulps of the
+// reference's unfused sum); only inside the +-1e-10 band around R, where the
+// two could disagree, does the reference's own expression and sqrt decide.
+// The order and the row storage are those of k_build_list.
+__global__ __launch_bounds__(kWave) void k_build_list_xrow(
+    Grid g, int n, double R, const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
     int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
     int flag_overflow, int flag_max)
 {
     __shared__ PosRec tile[kWave];
     __shared__ int32_t ring_all[kWave * kIdRing];
     int32_t* ring = ring_all + threadIdx.x * kIdRing;
-    const int b = xcd_block();
-    const long cellid = b / kCellHalves;
-    const int half = b % kCellHalves;
     const int lane = threadIdx.x;
-    const int s = start[cellid], m = start[cellid + 1] - s;
-    if (m <= half * kWave) return;                     // uniform over the block
-    const int ncxy = g.ncx * g.ncy;
-    const int ic = (int)(cellid % g.ncx) + 1;
-    const int jc = (int)((cellid / g.ncx) % g.ncy) + 1;
-    const int kc = (int)(cellid / ncxy) + 1;
-    const int m1 = (ic == 1) ? 0 : -1, m2 = (ic == g.ncx) ? 0 : 1;
-    const int m3 = (jc == 1) ? 0 : -1, m4 = (jc == g.ncy) ? 0 : 1;
-    int m5 = 0, m6 = 0;
-    if (g.dim == 3) { m5 = (kc == 1) ? 0 : -1; m6 = (kc == g.ncz) ? 0 : 1; }
+    const long t = (long)xcd_block() * kWave + lane;
+    const bool valid = t < n;
+    const PosRec me = rec[valid ? t : 0];
+    const int i = me.id;
+    const int key = me.aux;                           // cell, 0-based
+    const int ix = key % g.ncx;
+    const int rowkey = key / g.ncx;                   // y + ncy z
+    const long ncxy = (long)g.ncx * g.ncy;
+    const long rb = sl_index(i, 0, cap);
     const double R2 = R * R;
     const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
-    int kmax = 0;
-    for (int base = half * kWave; base < m; base += kCellHalves * kWave) {
-        const int t = base + lane;
-        const bool active = t < m;
-        const PosRec me = rec[s + (active ? t : 0)];
-        const int i = me.id;
-        const long rb = sl_index(i, 0, cap);
-        int k = 0;
-        for (int row = m1; row <= m2; ++row)
-            for (int col = m3; col <= m4; ++col)
-                for (int dep = m5; dep <= m6; ++dep) {
-                    const int cn = (ic - 1 + row) + (jc - 1 + col) * g.ncx + (kc - 1 + dep) * ncxy;
+    int k = 0;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+        const int lead = __ffsll(todo) - 1;
+        const int grow = __shfl(rowkey, lead);
+        const int gx = __shfl(ix, lead);
+        const bool mine = valid && rowkey == grow && ix <= gx + 1 && ((todo >> lane) & 1ull);
+        const unsigned long long m = __ballot(mine);
+        todo &= ~m;
+        const int last = 63 - __clzll(m);
+        const int x_lo = max(gx - 1, 0), x_hi = min(__shfl(ix, last) + 1, g.ncx - 1);
+        const int gy = grow % g.ncy, gz = grow / g.ncy;
+        const int y_lo = max(gy - 1, 0), y_hi = min(gy + 1, g.ncy - 1);
+        const int z_lo = (g.dim == 3) ? max(gz - 1, 0) : 0;
+        const int z_hi = (g.dim == 3) ? min(gz + 1, g.ncz - 1) : 0;
+        for (int xb = x_lo; xb <= x_hi; ++xb) {
+            const bool on = mine && xb >= ix - 1 && xb <= ix + 1;
+            for (int yb = y_lo; yb <= y_hi; ++yb)
+                for (int zb = z_lo; zb <= z_hi; ++zb) {
+                    const long cn = xb + yb * (long)g.ncx + zb * ncxy;
                     const int cs = start[cn], ce = start[cn + 1];
                     for (int j0 = cs; j0 < ce; j0 += kWave) {
                         const int nc = min(kWave, ce - j0);
@@ -468,35 +480,40 @@ __global__ __launch_bounds__(kWave) void k_build_list_cells(
                         if (lane < nc) tile[lane] = rec[j0 + lane];
                         __syncthreads();
                         // groups of 8 candidates: at most one group of 8
-                        // accepts completes per group
+                        // accepts completes per group. Slots past nc hold
+                        // stale records and are never accepted.
                         for (int u0 = 0; u0 < nc; u0 += 8) {
                             const int k0 = k;
 #pragma unroll
                             for (int uu = 0; uu < 8; ++uu) {
                                 const int u = u0 + uu;
-                                const PosRec cu = tile[u < nc ? u : 0];
-                                // z is 0 in 2D records: dx^2+dy^2+0 equals the 2D sum
+                                const PosRec cu = tile[u];
+                                // z is 0 in 2D records
                                 const double dx = cu.x - me.x, dy = cu.y - me.y, dz = cu.z - me.z;
-                                const double d2 = dx * dx + dy * dy + dz * dz;
-                                bool acc = d2 < lo;
-                                if (!acc && !(d2 > hi))   // rare band: exact sqrt decides
+                                const double d2f = fma(dz, dz, fma(dy, dy, dx * dx));
+                                bool acc = d2f < lo;
+                                if (!acc && !(d2f > hi)) {   // rare band: the reference's expression
+                                    const double d2 = dx * dx + dy * dy + dz * dz;
                                     acc = g.incl ? (sqrt(d2) <= R) : (sqrt(d2) < R);
-                                acc = acc && active && (u < nc) && (g.incl || cu.id != i);
-                                if (acc) {
-                                    ring[k & 15] = cu.id;
-                                    ++k;
                                 }
+                                acc = acc && on && (u < nc) && (g.incl || cu.id != i);
+                                // the next free slot takes every candidate; only
+                                // an accepted one advances past it
+                                ring[k & 15] = cu.id;
+                                k += acc ? 1 : 0;
                             }
                             if ((k0 >> 3) != (k >> 3)) flush_ids(ring, k0 >> 3, cap, ids + rb);
                         }
                     }
                 }
-        if (active && (k & 7)) flush_ids(ring, k >> 3, cap, ids + rb);
-        if (active) {
-            cnt[i] = k;
-            if (k > cap) atomicAdd(&flags[flag_overflow], 1);
-            kmax = max(kmax, k);
         }
+    }
+    if (valid && (k & 7)) flush_ids(ring, k >> 3, cap, ids + rb);
+    int kmax = 0;
+    if (valid) {
+        cnt[i] = k;
+        if (k > cap) atomicAdd(&flags[flag_overflow], 1);
+        kmax = k;
     }
     kmax = wave_max_i32(kmax);
     if (lane == 0) atomicMax(&flags[flag_max], kmax);
@@ -633,14 +650,13 @@ static int search_one(vox_ctx* c, int which, double re)
         ? L.g.hx * L.g.hx * L.g.hz / (c->p.dl * c->p.dl * c->p.dl)
         : L.g.hx * L.g.hx / (c->p.dl * c->p.dl);
     const int mode = c->tuned_mode[which] ? c->tuned_mode[which] : c->p.list_mode;
-    const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0)) &&
-                           L.g.tnc * kCellHalves < (1L << 31);
+    const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0));
     ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
     VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
     if (cell_walk) {
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-        k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
-            gg, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+        k_build_list_xrow<<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
+            gg, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
     } else {
         const int nb = grid_for(n);
         k_build_list<<<nb, kBlock, 0, c->stream>>>(
@@ -692,8 +708,8 @@ static int regrow_one(vox_ctx* c, int which, double re)
 
 // The reference rows of list `which` in place of compact ones, searched again
 // over the cell-sorted records of the NEIGHBOR call that built the list (the
-// positions of that call, whatever the particles did since), one wave per
-// cell, with wider rows while any overflows.
+// positions of that call, whatever the particles did since), by the cell-row
+// walk, with wider rows while any overflows.
 int ensure_reference_list(vox_ctx* c, int which)
 {
     DevList& L = c->list[which];
@@ -701,12 +717,12 @@ int ensure_reference_list(vox_ctx* c, int which)
     const int fo = which == 0 ? FLAG_OVERFLOW_S : FLAG_OVERFLOW_L;
     const int fm = which == 0 ? FLAG_MAXCNT_S : FLAG_MAXCNT_L;
     const double R = L.g.incl ? L.re + c->p.delta : L.re;
-    if (L.g.tnc * kCellHalves >= (1L << 31)) return fail(VOX_ERR_ARG, "neighbour grid too large");
+    const int n = c->tp;
     for (int attempt = 0; attempt < 4; ++attempt) {
         VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-        k_build_list_cells<<<(int)(L.g.tnc * kCellHalves), kWave, 0, c->stream>>>(
-            L.g, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+        k_build_list_xrow<<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
+            L.g, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
         VOX_HIP(hipGetLastError());
         VOX_TRY(read_flags(c));
         if (c->hflags[fo] == 0) {
@@ -741,7 +757,7 @@ static int build_one(vox_ctx* c, int which, double re)
 }
 
 // Tile-shape autotuner of the neighbour search: times both walks (one thread
-// per particle, one wave per cell) on the current positions, `reps` builds
+// per particle, one wave per 64 cell-sorted particles) on the current positions, `reps` builds
 // each, and keeps the faster one per list. The lists are identical either way,
 // so the choice only changes speed. modes[w] receives 1 or 2 per list.
 int autotune_lists(vox_ctx* c, int reps, int32_t* modes)
