@@ -477,6 +477,27 @@ __device__ __forceinline__ double wfun_r(double r, int ktype, double re, double 
     return 0.0;
 }
 
+// wfun_r for a kernel type fixed at compile time (KT = 2 or 6; KT = 0 reads
+// ktype at run time): the same expression, without the code of the other
+// types (the logarithm of type 5 alone holds dozens of registers in a
+// kernel that inlines it).
+template <int KT>
+__device__ __forceinline__ double wfun_k(double r, int ktype, double re, double inv_re)
+{
+    if constexpr (KT == 2) {
+        double q = div_by(r, re, inv_re);
+        return (q < 1.0) ? (1 / q) - 1 : 0.0;
+    } else if constexpr (KT == 6) {
+        double q = div_by(r, re, inv_re);
+        return (q <= 1.0) ? (1 - q) * (1 - q) * (1 - q) : 0.0;
+    } else {
+        return wfun_r(r, ktype, re, inv_re);
+    }
+}
+
+// The compile-time kernel type of a run (0: any other, read at run time).
+inline int kt_of(int ktype) { return (ktype == 2 || ktype == 6) ? ktype : 0; }
+
 // Fixed-order block sum of one double per thread (blockDim == kBlock).
 __device__ __forceinline__ double block_sum(double v, double* lds)
 {

@@ -80,7 +80,7 @@ constexpr int kRing = 17;
 // SV = double: FP64 values (and their FP32 copy when aval32 is set, for the
 // learned preconditioner's network); SV = float: mixed precision, FP32 values
 // only (computed in FP64 and rounded once; the diagonal stays FP64).
-template <class SV>
+template <class SV, int KT>
 __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
 {
     __shared__ int32_t ring_c[kBlock * kRing];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
                 // matopt: hl / d^3 / n0l / rho1 (functions.cu:903-913)
                 const double val = a.matopt
                     ? div_by(div_by(a.hl / (d * d * d), a.n0l, a.inv_n0l), a.rho1, a.inv_rho1)
-                    : div_by(div_by(a.c0 * wfun_r(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
+                    : div_by(div_by(a.c0 * wfun_k<KT>(d, a.ktype, a.re, a.inv_re), a.n0l, a.inv_n0l),
                              a.rho1, a.inv_rho1);
                 if (use && bj == 0 && val != 0.0) {
                     rc[k & 15] = rj.aux;
@@ -338,8 +338,14 @@ int ppe_assemble(vox_ctx* c)
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
     // rows: a grid for every particle, of which the blocks past nu (read on
     // the device) return at once
-    if (fp32_only) k_assemble_rows<float><<<grid_for(n), kBlock, 0, c->stream>>>(a);
-    else k_assemble_rows<double><<<grid_for(n), kBlock, 0, c->stream>>>(a);
+    switch (kt_of(p.ktype) * 2 + (fp32_only ? 1 : 0)) {
+    case 4: k_assemble_rows<double, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 5: k_assemble_rows<float, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 12: k_assemble_rows<double, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 13: k_assemble_rows<float, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 1: k_assemble_rows<float, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    default: k_assemble_rows<double, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    }
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];

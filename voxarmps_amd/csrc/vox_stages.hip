@@ -95,6 +95,9 @@ static void launch_pnd(vox_ctx* c, const DevList& L, double re, double ncorr, co
                        double* pnew)
 {
     k_pack_pos<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.dim, x, y, z, c->pos_rec);
+    // (a kernel specialised on the kernel type holds fewer registers, runs
+    // more waves and measured 11% slower: the gathers then share the caches
+    // among more rows)
     k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.ktype, re, ncorr,
                                                       ListView{L.cnt, L.ids, L.cap}, c->pos_rec,
                                                       out, p, pnew);
@@ -550,6 +553,8 @@ static void launch_pred_u(const PredArgs& a, int v, int n, hipStream_t s)
     }
 }
 
+// (variants specialised on the kernel type hold about 20 registers fewer and
+// measured the same or slower: the gathers, not the arithmetic, set the pace)
 static int launch_prediction(vox_ctx* c, const PredArgs& a, int variant)
 {
     if (uniform_meu(c)) launch_pred_u<true>(a, variant, c->tp, c->stream);
@@ -699,6 +704,7 @@ __global__ void k_pack_grad(GradArgs a)
 
 // Branch-free per entry with 8 records in flight; the 3-term distance equals
 // the 2D one bit for bit, and the z sum is only read in 3D.
+template <int KT>
 __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
 {
     int i = a.first + xcd_tid();
@@ -726,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_pressgrad(GradArgs a)
             const bool use = (jc[e] != i);
             const double dx = px[e] - xi, dy = py[e] - yi, dz = pz[e] - zi;
             const double D = sqrt(dx * dx + dy * dy + dz * dz);
-            const double wv = wfun_r(D, a.ktype, a.re, inv_re);
+            const double wv = wfun_k<KT>(D, a.ktype, a.re, inv_re);
             // kh 1: p_j + p_i - (p^_i + p^_j) (functions.cu:275-278); kh 2, the
             // reference's CUDA kernel: p_j + p_i - 2 p^_i (382-385); kh 0: p_j - p^_i
             const double dp = (a.kh == 1)   ? (pp[e] + pi_ - (phi_ + ph[e]))
@@ -773,7 +779,11 @@ int st_pressgrad(vox_ctx* c)
     int m = c->tp - a.first;
     if (m <= 0) return VOX_OK;
     k_pack_grad<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
-    k_pressgrad<<<grid_for(m), kBlock, 0, c->stream>>>(a);
+    switch (kt_of(a.ktype)) {
+    case 2: k_pressgrad<2><<<grid_for(m), kBlock, 0, c->stream>>>(a); break;
+    case 6: k_pressgrad<6><<<grid_for(m), kBlock, 0, c->stream>>>(a); break;
+    default: k_pressgrad<0><<<grid_for(m), kBlock, 0, c->stream>>>(a); break;
+    }
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
