@@ -102,6 +102,7 @@ struct DevList {
     int32_t* cell_fill = nullptr;  // tnc
     int32_t* order_raw = nullptr;  // tp (atomic scatter result)
     PosRec* rec = nullptr;         // tp: positions + id sorted by (cell, id)
+    bool sorted = false;           // rec holds a sort (a NEIGHBOR call has run)
     long tnc_alloc = 0;
     bool compact = false;          // rows hold only the entries of nonzero weight (see
                                    // list_compact_ok); the reference rows are rebuilt on demand
@@ -172,6 +173,14 @@ __device__ __forceinline__ void load_chunk8(const int32_t* __restrict__ row, int
     o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
 }
 inline long sl_size(long n, int cap) { return ((n + 63) / 64) * 64 * (long)cap; }
+
+// The particle a per-particle kernel's thread t takes: position t of a list's
+// cell-sorted order (a wave's particles then share most of their neighbours),
+// or particle t before the first sort.
+__device__ __forceinline__ int sorted_particle(const PosRec* order, int t)
+{
+    return order ? order[t].id : t;
+}
 
 // Lightweight by-value view of a list for kernels.
 struct ListView {
@@ -371,6 +380,9 @@ int build_neighbors(vox_ctx* c);
 bool list_compact_ok(const vox_ctx* c);
 int ensure_reference_list(vox_ctx* c, int which);
 int list_l_for_consumer(vox_ctx* c);
+// The cell-sorted records of list `which` as a per-particle kernels' thread
+// order (sorted_particle), or null before the first NEIGHBOR.
+const PosRec* list_order(const vox_ctx* c, int which);
 int ensure_list_storage(vox_ctx* c, int which, int cap);
 
 // stages (vox_stages.hip)

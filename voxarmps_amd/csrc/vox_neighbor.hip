@@ -341,19 +341,24 @@ __device__ __forceinline__ void flush_ids(const int32_t* ring, int c, int cap,
     o[1] = v4i{ring[s0 + 4], ring[s0 + 5], ring[s0 + 6], ring[s0 + 7]};
 }
 
+// Thread t takes the particle at position t of the cell-sorted order (its
+// record holds position, id and cell), so a wave's lanes sit in a few
+// neighbouring cells of one grid row: they walk the same or overlapping
+// stencils, with equal trip counts over a shared cell and candidate loads
+// that hit the same lines.
 __global__ __launch_bounds__(kBlock) void k_build_list(
-    Grid g, int n, double R, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const int32_t* __restrict__ cell,
-    const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
+    Grid g, int n, double R, const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
     int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
     int flag_overflow, int flag_max, int32_t* __restrict__ blockmax)
 {
     __shared__ int32_t ring_all[kBlock * kIdRing];
     int32_t* ring = ring_all + threadIdx.x * kIdRing;
-    const int i = xcd_tid();
+    const int t = xcd_tid();
     int k = 0;
-    if (i < n) {
-        const int32_t key = cell[i];
+    if (t < n) {
+        const PosRec me = rec[t];
+        const int i = me.id;
+        const int32_t key = me.aux;
         const int ncxy = g.ncx * g.ncy;
         const int ic = key % g.ncx + 1;
         const int jc = (key / g.ncx) % g.ncy + 1;
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
         const int m3 = (jc == 1) ? 0 : -1, m4 = (jc == g.ncy) ? 0 : 1;
         int m5 = 0, m6 = 0;
         if (g.dim == 3) { m5 = (kc == 1) ? 0 : -1; m6 = (kc == g.ncz) ? 0 : 1; }
-        const double xi = x[i], yi = y[i], zi = (g.dim == 3) ? z[i] : 0.0;
+        const double xi = me.x, yi = me.y, zi = me.z;
         const double R2 = R * R;
         const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
         const long rb = sl_index(i, 0, cap);
@@ -563,6 +568,12 @@ static int search_one(vox_ctx* c, int which, double re);
 // by re/d^3 at any d and need the reference rows, as does an export:
 // ensure_reference_list rebuilds them from the cell-sorted records of the same
 // NEIGHBOR call.
+const PosRec* list_order(const vox_ctx* c, int which)
+{
+    const DevList& L = list_of(c, which);
+    return L.sorted ? L.rec : nullptr;
+}
+
 bool list_compact_ok(const vox_ctx* c)
 {
     const vox_params& p = c->p;
@@ -626,6 +637,7 @@ static int launch_one(vox_ctx* c, int which, double re)
         }
         VOX_HIP(hipGetLastError());
     }
+    L.sorted = true;
     return search_one(c, which, re);
 }
 
@@ -659,9 +671,8 @@ static int search_one(vox_ctx* c, int which, double re)
             gg, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
     } else {
         const int nb = grid_for(n);
-        k_build_list<<<nb, kBlock, 0, c->stream>>>(
-            gg, n, R, c->f[VOX_F_X], c->f[VOX_F_Y], c->f[VOX_F_Z], L.cell, L.cell_start,
-            L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm, c->blockmax);
+        k_build_list<<<nb, kBlock, 0, c->stream>>>(gg, n, R, L.cell_start, L.rec, L.cap, L.ids,
+                                                    L.cnt, c->dflags, fo, fm, c->blockmax);
         k_reduce_max<<<1, kBlock, 0, c->stream>>>(c->blockmax, nb, c->dflags + fm);
     }
     VOX_HIP(hipGetLastError());

@@ -59,12 +59,17 @@ __global__ void k_pack_pos(int n, int dim, const double* __restrict__ x,
 // bit for bit (dz = 0).
 __global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, double ncorr,
                                                 ListView L, const PosRec* __restrict__ rec,
+                                                const PosRec* __restrict__ order,
                                                 double* __restrict__ out,
                                                 const double* __restrict__ p,
                                                 double* __restrict__ pnew)
 {
-    const int i = xcd_tid();
-    if (i >= n) return;
+    const int t = xcd_tid();
+    if (t >= n) return;
+    // (in the list's cell order the rows, stored by particle id, are read
+    // from scattered slices: measured 20% slower than particle order here,
+    // where the gathers are light and the row reads dominate)
+    const int i = sorted_particle(order, t);
     const PosRec me = rec[i];
     double sum = 0.0;
     const double inv_re = 1.0 / re;
@@ -100,7 +105,7 @@ static void launch_pnd(vox_ctx* c, const DevList& L, double re, double ncorr, co
     // among more rows)
     k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.ktype, re, ncorr,
                                                       ListView{L.cnt, L.ids, L.cap}, c->pos_rec,
-                                                      out, p, pnew);
+                                                      nullptr, out, p, pnew);
 }
 
 int st_pnd(vox_ctx* c, int on_star, int loopstarted)
@@ -415,6 +420,7 @@ struct PredArgs {
     const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
     double *xs, *ys, *zs, *us, *vs, *ws;
     PredRec* rec;
+    const PosRec* order;      // thread t takes particle order[t].id (cell-sorted)
 };
 
 // One 64-byte record per particle with every field the predictor gathers.
@@ -440,8 +446,9 @@ __global__ void k_pack_pred(PredArgs a)
 template <bool kUniform, int R, int BS>
 __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
 {
-    int i = xcd_tid();
-    if (i >= a.n) return;
+    const int t = xcd_tid();
+    if (t >= a.n) return;
+    const int i = sorted_particle(a.order, t);
     const int dim = a.dim;
     const int ti = a.pt[i];
     if (ti <= 0) {
@@ -595,6 +602,9 @@ int st_prediction(vox_ctx* c)
     a.xs = c->f[VOX_F_XSTAR]; a.ys = c->f[VOX_F_YSTAR]; a.zs = c->f[VOX_F_ZSTAR];
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = c->pred_rec;
+    // rows in list L's cell order: a wave's particles share most neighbours
+    // (2.5% faster than particle order, whose row reads are contiguous)
+    a.order = list_order(c, VOX_LIST_L);
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_TRY(launch_prediction(c, a, c->tune_pred));
     return VOX_OK;
