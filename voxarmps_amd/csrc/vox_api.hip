@@ -434,7 +434,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         if (w == 1 && !c->two_lists) break;
         DevList& L = c->list[w];
         if ((rc = dalloc(&L.cnt, n)) || (rc = dalloc(&L.cell, n)) || (rc = dalloc(&L.order_raw, n)) ||
-            (rc = dalloc(&L.rec, n)))
+            (rc = dalloc(&L.rec, n)) || (rc = dalloc(&L.pos_of, n)))
             return bail(rc);
         double re = (w == 0) ? c->p.re_s : c->p.re_l;
         int want = (w == 0 ? c->p.cap_s : c->p.cap_l);
@@ -444,7 +444,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
     }
     for (int k = 0; k < 6; ++k)
         if ((rc = dalloc(&c->sps_s[k], n))) return bail(rc);
-    if ((rc = dalloc(&c->scratch_d, n)) || (rc = dalloc(&c->pred_rec, n)) ||
+    if ((rc = dalloc(&c->scratch_d, n)) || (rc = dalloc(&c->pred_rec, n)) || (rc = dalloc(&c->pt_sorted, n)) ||
         (rc = dalloc(&c->pos_rec, n)) || (rc = dalloc(&c->blockmax, (size_t)grid_for(n))) ||
         (rc = dalloc(&c->blockmax2, (size_t)grid_for(n))))
         return bail(rc);
@@ -482,10 +482,10 @@ int vox_destroy(vox_ctx* c)
     for (int w = 0; w < 2; ++w) {
         DevList& L = c->list[w];
         dfree(L.cnt); dfree(L.ids); dfree(L.cell); dfree(L.cell_start); dfree(L.cell_fill);
-        dfree(L.order_raw); dfree(L.rec);
+        dfree(L.order_raw); dfree(L.rec); dfree(L.pos_of);
     }
     for (int k = 0; k < 6; ++k) dfree(c->sps_s[k]);
-    dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pos_rec); dfree(c->blockmax); dfree(c->blockmax2);
+    dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pt_sorted); dfree(c->pos_rec); dfree(c->blockmax); dfree(c->blockmax2);
     dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->rowlen); dfree(c->acol);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);

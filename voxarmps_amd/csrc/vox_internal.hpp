@@ -107,6 +107,9 @@ struct DevList {
     bool compact = false;          // rows hold only the entries of nonzero weight (see
                                    // list_compact_ok); the reference rows are rebuilt on demand
     uint64_t epoch = 0;            // vox_ctx::pos_epoch when the rows were searched
+    int32_t* pos_of = nullptr;     // tp: particle id -> its position in rec
+    bool pos_rows = false;         // row entries are positions in rec, not particle ids
+                                   // (list L's compact FP64 rows; see list_gather)
 };
 
 // Sliced row storage in chunks of 8 (SELL-64x8). Rows are grouped in slices
@@ -221,6 +224,7 @@ struct vox_ctx {
     double* sps_s[6] = {};
     double* scratch_d = nullptr;   // tp doubles
     vox::PredRec* pred_rec = nullptr;  // tp gather records of the predictor
+    int32_t* pt_sorted = nullptr;  // tp: PTYPE in list L's cell order (predictor)
     vox::PosRec* pos_rec = nullptr;    // tp gather records (positions + two ints)
 
     // PPE (rows are unknowns r = uidx[i]; SELL-64 rows, mx_index)
@@ -380,6 +384,7 @@ int build_neighbors(vox_ctx* c);
 bool list_compact_ok(const vox_ctx* c);
 int ensure_reference_list(vox_ctx* c, int which);
 int list_l_for_consumer(vox_ctx* c);
+int list_l_ids(vox_ctx* c);     // list L's rows as particle ids (consumers that need them)
 // The cell-sorted records of list `which` as a per-particle kernels' thread
 // order (sorted_particle), or null before the first NEIGHBOR.
 const PosRec* list_order(const vox_ctx* c, int which);

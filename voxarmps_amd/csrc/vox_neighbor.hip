@@ -258,8 +258,8 @@ constexpr int kSortTile = 512;   // ids per wave held in LDS
 
 __global__ __launch_bounds__(kBlock) void k_cell_sort(
     long tnc, int dim, const int32_t* __restrict__ start, const int32_t* __restrict__ order_raw,
-    PosRec* __restrict__ rec, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z)
+    PosRec* __restrict__ rec, int32_t* __restrict__ pos_of, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z)
 {
     __shared__ int32_t tile[kBlock / kWave][kSortTile];
     int wv = threadIdx.x / kWave, lane = threadIdx.x % kWave;
@@ -288,6 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
         rr.id = id;
         rr.aux = (int32_t)cellid;
         rec[pos] = rr;
+        pos_of[id] = pos;
     }
 }
 
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort(
 // cell's members below its own, which is its rank in the ascending run.
 __global__ __launch_bounds__(kBlock) void k_cell_rank(
     int n, int dim, const int32_t* __restrict__ cell, const int32_t* __restrict__ start,
-    const int32_t* __restrict__ order_raw, PosRec* __restrict__ rec,
+    const int32_t* __restrict__ order_raw, PosRec* __restrict__ rec, int32_t* __restrict__ pos_of,
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,6 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(
     rr.id = id;
     rr.aux = c;
     rec[s + rank] = rr;
+    pos_of[id] = s + rank;
 }
 
 // Per-particle stencil walk. R is the accept radius (3D and the CUDA path:
@@ -438,6 +440,8 @@ __global__ __launch_bounds__(kBlock) void k_build_list(
 // reference's unfused sum); only inside the +-1e-10 band around R, where the
 // two could disagree, does the reference's own expression and sqrt decide.
 // The order and the row storage are those of k_build_list.
+// kPos: the rows hold each neighbour's position in rec instead of its id.
+template <bool kPos>
 __global__ __launch_bounds__(kWave) void k_build_list_xrow(
     Grid g, int n, double R, const int32_t* __restrict__ start, const PosRec* __restrict__ rec,
     int cap, int32_t* __restrict__ ids, int32_t* __restrict__ cnt, int32_t* __restrict__ flags,
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
                                 acc = acc && on && (u < nc) && (g.incl || cu.id != i);
                                 // the next free slot takes every candidate; only
                                 // an accepted one advances past it
-                                ring[k & 15] = cu.id;
+                                ring[k & 15] = kPos ? j0 + u : cu.id;
                                 k += acc ? 1 : 0;
                             }
                             if ((k0 >> 3) != (k >> 3)) flush_ids(ring, k0 >> 3, cap, ids + rb);
@@ -627,12 +631,14 @@ static int launch_one(vox_ctx* c, int which, double re)
         // cells of fewer than ~24 expected particles: one thread per particle
         if (few_per_cell(c, L)) {
             k_cell_rank<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->p.dim, L.cell, L.cell_start,
-                                                                L.order_raw, L.rec, c->f[VOX_F_X],
+                                                                L.order_raw, L.rec, L.pos_of,
+                                                                c->f[VOX_F_X],
                                                                 c->f[VOX_F_Y], c->f[VOX_F_Z]);
         } else {
             long nb = (L.g.tnc + (kBlock / kWave) - 1) / (kBlock / kWave);
             k_cell_sort<<<(int)nb, kBlock, 0, c->stream>>>(L.g.tnc, c->p.dim, L.cell_start,
-                                                            L.order_raw, L.rec, c->f[VOX_F_X],
+                                                            L.order_raw, L.rec, L.pos_of,
+                                                            c->f[VOX_F_X],
                                                             c->f[VOX_F_Y], c->f[VOX_F_Z]);
         }
         VOX_HIP(hipGetLastError());
@@ -665,10 +671,18 @@ static int search_one(vox_ctx* c, int which, double re)
     const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0));
     ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
     VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
+    // Compact FP64 rows of the cell walk hold positions in the cell-sorted
+    // records: their consumers gather from records in that order, where a
+    // row's consecutive entries (one cell's particles) sit side by side.
+    L.pos_rows = compact && cell_walk && c->p.precision == VOX_PREC_FP64;
     if (cell_walk) {
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-        k_build_list_xrow<<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
-            gg, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+        if (L.pos_rows)
+            k_build_list_xrow<true><<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
+                gg, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
+        else
+            k_build_list_xrow<false><<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
+                gg, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
     } else {
         const int nb = grid_for(n);
         k_build_list<<<nb, kBlock, 0, c->stream>>>(gg, n, R, L.cell_start, L.rec, L.cap, L.ids,
@@ -732,9 +746,10 @@ int ensure_reference_list(vox_ctx* c, int which)
     for (int attempt = 0; attempt < 4; ++attempt) {
         VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
-        k_build_list_xrow<<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
+        k_build_list_xrow<false><<<(n + kWave - 1) / kWave, kWave, 0, c->stream>>>(
             L.g, n, R, L.cell_start, L.rec, L.cap, L.ids, L.cnt, c->dflags, fo, fm);
         VOX_HIP(hipGetLastError());
+        L.pos_rows = false;
         VOX_TRY(read_flags(c));
         if (c->hflags[fo] == 0) {
             L.compact = false;
@@ -744,6 +759,14 @@ int ensure_reference_list(vox_ctx* c, int which)
         VOX_TRY(ensure_list_storage(c, which, ((need + need / 4) + 7) & ~7));
     }
     return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
+}
+
+// List L's rows as particle ids: rows of positions are replaced by the
+// reference rows (searched over the same records).
+int list_l_ids(vox_ctx* c)
+{
+    if (c->two_lists && c->list[1].pos_rows) return ensure_reference_list(c, 1);
+    return VOX_OK;
 }
 
 // Called by every consumer of list L before it reads the rows: compact rows

@@ -47,22 +47,26 @@ struct AsmArgs {
     int32_t *acol, *rowlen;
     double *aval, *adiag, *rhs, *source;
     float* aval32;                         // FP32 values (mixed precision, learned network)
-    PosRec* rec;
+    PosRec* rec;                           // the records the row entries index
+    const PosRec* order;                   // rows of positions: list L's sorted records
+    const int32_t* pos_of;                 //   and each particle's position there (else null)
 };
 
 // 32-byte gather record per particle: position, bcon (id slot), unknown
-// index; and the source term of the default MATRIX (functions.cu:990-1001),
-// 0 for every particle that is not an unknown.
+// index - at the particle's id, or at its position in list L's sorted records
+// when the rows hold positions; and the source term of the default MATRIX
+// (functions.cu:990-1001), 0 for every particle that is not an unknown.
 __global__ void k_pack_asm(AsmArgs a)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int i = a.order ? a.order[t].id : t;
     PosRec r;
     r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
     const int bi = a.bcon[i];
     r.id = bi;
     r.aux = a.uidx[i];
-    a.rec[i] = r;
+    a.rec[t] = r;
     a.source[i] = (bi == 0) ? a.inv_dt_dt * (a.pnd[i] - a.n0s) / a.n0s : 0.0;
 }
 
@@ -90,10 +94,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = r < nu;
     const int i = live ? a.urow[r] : 0;
+    const int self = a.pos_of ? a.pos_of[i] : i;   // own index among the entries' records
     int32_t* rc = ring_c + threadIdx.x * kRing;
     double* rv = ring_v + threadIdx.x * kRing;
     const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
-    const PosRec me = a.rec[i];
+    const PosRec me = a.rec[self];
     const double xi = me.x, yi = me.y, zi = me.z;
     const long ob = mx_index(r, 0, a.acap);                  // entry k at ob + 64 k
     double diag = 0.0;
@@ -109,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
             PosRec rr[U];
             load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-            for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+            for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : self;
 #pragma unroll
             for (int u = 0; u < U; ++u) rr[u] = a.rec[jj[u]];
             // Branch-free per entry (a skipped entry keeps diag unchanged
@@ -123,8 +128,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
                 // unchanged, so one expression serves both dimensions
                 const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
                 const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                // jj == i: self, or padding past the row end
-                const bool use = (jj[u] != i) && (bj != -1) && !(fabs(d) <= 1.0e-15);
+                // jj == self: the particle itself, or padding past the row end
+                const bool use = (jj[u] != self) && (bj != -1) && !(fabs(d) <= 1.0e-15);
                 // (= c0 * W(d) / n0l / rho1 with IEEE divisions, see div_by);
                 // matopt: hl / d^3 / n0l / rho1 (functions.cu:903-913)
                 const double val = a.matopt
@@ -335,6 +340,11 @@ int ppe_assemble(vox_ctx* c)
     a.adiag = c->adiag; a.rhs = c->rhs;
     a.source = c->f[VOX_F_SOURCE];
     a.rec = c->pos_rec;
+    // rows of positions (list L's compact rows) index list L's sorted records,
+    // whose positions are this assembly's (the CPU path's x, unchanged since
+    // the search: list_l_for_consumer)
+    a.order = L.pos_rows ? L.rec : nullptr;
+    a.pos_of = L.pos_rows ? L.pos_of : nullptr;
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
     // rows: a grid for every particle, of which the blocks past nu (read on
     // the device) return at once

@@ -57,20 +57,23 @@ __global__ void k_pack_pos(int n, int dim, const double* __restrict__ x,
 // n_i = sum over the row of W(d_ij), j != i, in row order. Branch-free per
 // entry with 8 record loads in flight; the 3-term distance equals the 2D one
 // bit for bit (dz = 0).
+// rec: the records the row entries index (by particle id, or by position in
+// the list's cell-sorted records when self_of is given: self_of[i] is then
+// particle i's own position there).
 __global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, double ncorr,
                                                 ListView L, const PosRec* __restrict__ rec,
-                                                const PosRec* __restrict__ order,
+                                                const int32_t* __restrict__ self_of,
                                                 double* __restrict__ out,
                                                 const double* __restrict__ p,
                                                 double* __restrict__ pnew)
 {
-    const int t = xcd_tid();
-    if (t >= n) return;
-    // (in the list's cell order the rows, stored by particle id, are read
+    // (threads in the list's cell order read the rows, stored by particle id,
     // from scattered slices: measured 20% slower than particle order here,
     // where the gathers are light and the row reads dominate)
-    const int i = sorted_particle(order, t);
-    const PosRec me = rec[i];
+    const int i = xcd_tid();
+    if (i >= n) return;
+    const int self = self_of ? self_of[i] : i;
+    const PosRec me = rec[self];
     double sum = 0.0;
     const double inv_re = 1.0 / re;
     const int cnt = L.cnt[i];
@@ -80,14 +83,14 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, dou
         load_chunk8(rowp, g, jc);
         PosRec rr[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) jc[u] = ((g << 3) + u < cnt) ? jc[u] : i;
+        for (int u = 0; u < 8; ++u) jc[u] = ((g << 3) + u < cnt) ? jc[u] : self;
 #pragma unroll
         for (int u = 0; u < 8; ++u) rr[u] = rec[jc[u]];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const double dx = rr[u].x - me.x, dy = rr[u].y - me.y, dz = rr[u].z - me.z;
             const double w = wfun_r(sqrt(dx * dx + dy * dy + dz * dz), ktype, re, inv_re);
-            sum = (jc[u] != i) ? sum + w : sum;
+            sum = (jc[u] != self) ? sum + w : sum;
         }
     }
     if (ktype != 2) sum = sum * ncorr;
@@ -95,17 +98,26 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, dou
     if (pnew) pnew[i] = p[i];
 }
 
+// Rows of positions (list L's compact rows) index the list's own sorted
+// records, which hold the positions of its search: the caller passes those
+// positions (n0 right after NEIGHBOR).
 static void launch_pnd(vox_ctx* c, const DevList& L, double re, double ncorr, const double* x,
                        const double* y, const double* z, double* out, const double* p,
                        double* pnew)
 {
-    k_pack_pos<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.dim, x, y, z, c->pos_rec);
+    const PosRec* rec = L.rec;
+    const int32_t* self_of = L.pos_of;
+    if (!L.pos_rows) {
+        k_pack_pos<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.dim, x, y, z, c->pos_rec);
+        rec = c->pos_rec;
+        self_of = nullptr;
+    }
     // (a kernel specialised on the kernel type holds fewer registers, runs
     // more waves and measured 11% slower: the gathers then share the caches
     // among more rows)
     k_pnd<<<grid_for(c->tp), kBlock, 0, c->stream>>>(c->tp, c->p.ktype, re, ncorr,
-                                                      ListView{L.cnt, L.ids, L.cap}, c->pos_rec,
-                                                      nullptr, out, p, pnew);
+                                                      ListView{L.cnt, L.ids, L.cap}, rec, self_of,
+                                                      out, p, pnew);
 }
 
 int st_pnd(vox_ctx* c, int on_star, int loopstarted)
@@ -419,20 +431,28 @@ struct PredArgs {
     const int32_t* pt;
     const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
     double *xs, *ys, *zs, *us, *vs, *ws;
-    PredRec* rec;
+    PredRec* rec;             // the records the row entries index (see pos)
     const PosRec* order;      // thread t takes particle order[t].id (cell-sorted)
+    int32_t* rpt;             // pos: the types in the same order as rec
+    bool pos;                 // row entries are positions in order (list L's compact
+                              // rows): rec and rpt are in that order, and thread t's
+                              // own record is rec[t]; else both are by particle id
 };
 
-// One 64-byte record per particle with every field the predictor gathers.
+// One 64-byte record per particle with every field the predictor gathers, at
+// the particle's id, or at its position in the cell-sorted order (with its
+// type beside it) when the rows hold positions.
 __global__ void k_pack_pred(PredArgs a)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int i = a.pos ? a.order[t].id : t;
+    if (a.pos) a.rpt[t] = a.pt[i];
     PredRec r;
     r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
     r.u = a.u[i]; r.v = a.v[i]; r.w = (a.dim == 3) ? a.w[i] : 0.0;
     r.pn = a.pn[i]; r.meu = a.meu[i];
-    a.rec[i] = r;
+    a.rec[t] = r;
 }
 
 // kUniform: every particle has the same MEU m (Newtonian, no fluid 2) and
@@ -449,6 +469,8 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
     const int t = xcd_tid();
     if (t >= a.n) return;
     const int i = sorted_particle(a.order, t);
+    const int self = a.pos ? t : i;          // own index among the row entries' records
+    const int32_t* __restrict__ rpt = a.pos ? a.rpt : a.pt;
     const int dim = a.dim;
     const int ti = a.pt[i];
     if (ti <= 0) {
@@ -456,7 +478,7 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
         if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
         return;
     }
-    const PredRec me = a.rec[i];
+    const PredRec me = a.rec[self];
     const double xi = me.x, yi = me.y, zi = me.z, ui = me.u, vi = me.v, wi = me.w;
     const double mi = me.meu, phi_ = a.ph[i];
     double rhoN = (ti == 1) ? a.rho1 : a.rho2;
@@ -477,11 +499,11 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
     int jj[U], tj[U];
     PredRec rv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + U * h + u < cnt) ? jc[U * h + u] : i;
+    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + U * h + u < cnt) ? jc[U * h + u] : self;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         rv[u] = a.rec[jj[u]];
-        tj[u] = kUniform ? ti : a.pt[jj[u]];
+        tj[u] = kUniform ? ti : rpt[jj[u]];
     }
     // Branch-free per neighbour: every term is computed and a skipped entry
     // (self, padding, coincident particle) keeps each sum through a select, so
@@ -493,7 +515,7 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
         // bit for bit, and no load is left inside a dimension branch
         const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
-        const bool use = (jj[u] != i) && (D > 0);
+        const bool use = (jj[u] != self) && (D > 0);
         const double mj = rj.meu;
         // every quotient below equals the plain IEEE division (div_by); the
         // six divisions by D share one reciprocal
@@ -582,7 +604,10 @@ const char* pred_variant_name(int v)
 int st_prediction(vox_ctx* c)
 {
     VOX_TRY(list_l_for_consumer(c));
-    if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
+    if (c->p.precision == VOX_PREC_MIXED) {
+        VOX_TRY(list_l_ids(c));           // the FP32 predictor reads particle ids
+        return st_prediction32(c);
+    }
     ProfScope ps(c, KC_PRED);
     const vox_params& p = c->p;
     PredArgs a;
@@ -605,6 +630,8 @@ int st_prediction(vox_ctx* c)
     // rows in list L's cell order: a wave's particles share most neighbours
     // (2.5% faster than particle order, whose row reads are contiguous)
     a.order = list_order(c, VOX_LIST_L);
+    a.pos = list_of(c, VOX_LIST_L).pos_rows && a.order;
+    a.rpt = c->pt_sorted;
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_TRY(launch_prediction(c, a, c->tune_pred));
     return VOX_OK;
