@@ -448,6 +448,7 @@ int vox_create(vox_ctx** out, const vox_params* pin, int tp, int gp, int wp, int
         (rc = dalloc(&c->pos_rec, n)) || (rc = dalloc(&c->blockmax, (size_t)grid_for(n))) ||
         (rc = dalloc(&c->blockmax2, (size_t)grid_for(n))))
         return bail(rc);
+    if ((rc = dalloc(&c->uorder, n)) || (rc = dalloc(&c->uscan, n))) return bail(rc);
     if ((rc = dalloc(&c->uidx, n)) || (rc = dalloc(&c->urow, n)) || (rc = dalloc(&c->uflag, n)) ||
         (rc = dalloc(&c->rowlen, n)) || (rc = dalloc(&c->adiag, n)) || (rc = dalloc(&c->rhs, n)) ||
         (rc = dalloc(&c->cg_x, n)) || (rc = dalloc(&c->cg_r, n)) || (rc = dalloc(&c->cg_p, n)) ||
@@ -486,7 +487,7 @@ int vox_destroy(vox_ctx* c)
     }
     for (int k = 0; k < 6; ++k) dfree(c->sps_s[k]);
     dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pt_sorted); dfree(c->pos_rec); dfree(c->blockmax); dfree(c->blockmax2);
-    dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->rowlen); dfree(c->acol);
+    dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->uorder); dfree(c->uscan); dfree(c->zcell); dfree(c->rowlen); dfree(c->acol);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
     dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);

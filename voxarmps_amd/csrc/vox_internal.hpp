@@ -232,6 +232,12 @@ struct vox_ctx {
     int32_t* urow = nullptr;       // tp: unknown -> particle
     int32_t* uflag = nullptr;      // tp scan input
     int32_t* rowlen = nullptr;     // tp: stored off-diagonal entries of each row
+    int32_t* uorder = nullptr;     // tp: particles in the numbering's cell order
+    int32_t* uscan = nullptr;      // tp: unknown index at each position of uorder
+    int32_t* zcell = nullptr;      // 2 (tnc + 1): cell counts and starts, z fastest
+    long zcell_alloc = 0;
+    bool uorder_cells = false;     // unknowns numbered in cell order (else particle order)
+    bool uorder_particles = false; // force particle order (a dump for the learned network)
     int32_t* acol = nullptr;       // cap_l * tp, mx_index
     double* aval = nullptr;        // cap_l * tp, mx_index
     double* adiag = nullptr;       // tp

@@ -1107,6 +1107,14 @@ int vox_lpc_dump(vox_lpc* L, vox_ctx* c, int32_t* index)
     if (!c) return fail(VOX_ERR_ARG, "null context");
     if (c->device != L->device) return fail(VOX_ERR_ARG, "context and network on different devices");
     if (!c->acol || c->nu <= 0) return fail(VOX_ERR_STATE, "no assembled PPE in the context");
+    if (c->uorder_cells) {
+        // training systems are dumped with the unknowns in particle order, the
+        // order of vox_ppe_export: the system is assembled again in that order
+        c->uorder_particles = true;
+        const int rc0 = ppe_assemble(c);
+        c->uorder_particles = false;
+        VOX_TRY(rc0);
+    }
     if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
     LpcSys S;
     S.nu = c->nu; S.rows = round64(c->nu); S.cap = c->a_cap;

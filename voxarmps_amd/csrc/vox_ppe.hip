@@ -35,6 +35,90 @@ __global__ void k_umap(int n, const int32_t* __restrict__ bcon, int32_t* __restr
     }
 }
 
+// ---- numbering of the unknowns -------------------------------------------------------
+// The unknowns are the bcon = 0 particles, numbered in the cell order of list
+// L's grid with z fastest (then y, then x; inside a cell by particle id), so
+// the assembly's rows and the solver's vectors run through space cell by
+// cell: a wave's 64 rows are neighbours, and so are most of their columns.
+// The reference numbers them in particle order (functions.cu:869-925); the
+// export maps back to it. Before the first NEIGHBOR, and for a dump of the
+// learned network's training systems, particle order is used.
+__device__ __forceinline__ int zkey(const Grid& g, int c)
+{
+    const int ix = c % g.ncx, iy = (c / g.ncx) % g.ncy, iz = c / (g.ncx * g.ncy);
+    return iz + g.ncz * (iy + g.ncy * ix);
+}
+
+__global__ void k_zcount(Grid g, const int32_t* __restrict__ start, int32_t* __restrict__ zcnt)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < g.tnc) zcnt[zkey(g, (int)c)] = start[c + 1] - start[c];
+}
+
+// Position t of the list's cell-sorted records goes to its cell's start in
+// z-fastest order plus its rank inside the cell.
+__global__ void k_uorder(int n, Grid g, const PosRec* __restrict__ rec,
+                         const int32_t* __restrict__ start, const int32_t* __restrict__ zstart,
+                         const int32_t* __restrict__ bcon, int32_t* __restrict__ uflag,
+                         int32_t* __restrict__ uorder)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const PosRec r = rec[t];
+    const int np = zstart[zkey(g, r.aux)] + (t - start[r.aux]);
+    uorder[np] = r.id;
+    uflag[np] = (bcon[r.id] == 0) ? 1 : 0;
+}
+
+__global__ void k_umap_order(int n, const int32_t* __restrict__ uorder,
+                             const int32_t* __restrict__ uflag, const int32_t* __restrict__ uscan,
+                             int32_t* __restrict__ uidx, int32_t* __restrict__ urow)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int id = uorder[t];
+    if (uflag[t]) {
+        const int r = uscan[t];
+        uidx[id] = r;
+        urow[r] = id;
+    } else {
+        uidx[id] = -1;
+    }
+}
+
+static int number_unknowns(vox_ctx* c)
+{
+    const int n = c->tp;
+    const DevList& L = list_of(c, VOX_LIST_L);
+    if (c->uorder_particles || !L.sorted) {
+        k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
+        VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
+        k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
+        VOX_HIP(hipGetLastError());
+        c->uorder_cells = false;
+        return VOX_OK;
+    }
+    const long tnc = L.g.tnc;
+    if (c->zcell_alloc < tnc) {
+        if (c->zcell) (void)hipFree(c->zcell);
+        c->zcell = nullptr;
+        VOX_HIP(hipMalloc(&c->zcell, sizeof(int32_t) * 2 * (size_t)(tnc + 1)));
+        c->zcell_alloc = tnc;
+    }
+    int32_t* zcnt = c->zcell;
+    int32_t* zstart = c->zcell + (tnc + 1);
+    k_zcount<<<grid_for(tnc), kBlock, 0, c->stream>>>(L.g, L.cell_start, zcnt);
+    VOX_TRY(scan_exclusive(c, zcnt, zstart, tnc, zstart + tnc));
+    k_uorder<<<grid_for(n), kBlock, 0, c->stream>>>(n, L.g, L.rec, L.cell_start, zstart, c->bcon,
+                                                     c->uflag, c->uorder);
+    VOX_TRY(scan_exclusive(c, c->uflag, c->uscan, n, c->dflags + FLAG_NU));
+    k_umap_order<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->uorder, c->uflag, c->uscan, c->uidx,
+                                                         c->urow);
+    VOX_HIP(hipGetLastError());
+    c->uorder_cells = true;
+    return VOX_OK;
+}
+
 struct AsmArgs {
     int n, dim, ktype, acap, matopt;
     double re, c0, n0l, rho1, dshift, inv_dt_dt, n0s;
@@ -313,9 +397,7 @@ int ppe_assemble(vox_ctx* c)
     const bool mixed = fp32_only || (c->p.solver == VOX_SOLVER_LEARNED_FCG && c->lpc);
     if (mixed && !c->aval32)
         VOX_HIP(hipMalloc(&c->aval32, sizeof(float) * (size_t)sl_size(c->tp, c->a_cap)));
-    k_uflag<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uflag);
-    VOX_TRY(scan_exclusive(c, c->uflag, c->uidx, n, c->dflags + FLAG_NU));
-    k_umap<<<grid_for(n), kBlock, 0, c->stream>>>(n, c->bcon, c->uidx, c->urow);
+    VOX_TRY(number_unknowns(c));
     const vox_params& p = c->p;
     AsmArgs a;
     a.n = n; a.dim = p.dim; a.ktype = p.ktype; a.re = p.re_l;
@@ -471,20 +553,29 @@ int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, i
         VOX_HIP(hipMemcpy(b.data(), c->rhs, sizeof(double) * nu, hipMemcpyDeviceToHost));
         VOX_HIP(hipMemcpy(urow.data(), c->urow, sizeof(int32_t) * nu, hipMemcpyDeviceToHost));
     }
+    // rows and columns in the reference's numbering (unknowns in particle
+    // order): row o of the export is the unknown of the o-th smallest particle
+    std::vector<int32_t> row_of(nu > 0 ? nu : 1), num(nu > 0 ? nu : 1);
+    for (int r = 0; r < nu; ++r) row_of[r] = r;
+    if (c->uorder_cells)
+        std::sort(row_of.begin(), row_of.begin() + nu,
+                  [&](int32_t a, int32_t b2) { return urow[a] < urow[b2]; });
+    for (int o = 0; o < nu; ++o) num[row_of[o]] = o;
     long k = 0;
-    for (int r = 0; r < nu; ++r) {
-        if (rowptr) rowptr[r] = (int32_t)k;
-        if (cols) cols[k] = r;
+    for (int o = 0; o < nu; ++o) {
+        const int r = row_of[o];
+        if (rowptr) rowptr[o] = (int32_t)k;
+        if (cols) cols[k] = o;
         if (vals) vals[k] = diag[r];
         ++k;
         for (int s = 0; s < len[r]; ++s) {
             const double v = aval[mx_index(r, s, c->a_cap)];
-            if (cols) cols[k] = acol[mx_index(r, s, c->a_cap)];
+            if (cols) cols[k] = num[acol[mx_index(r, s, c->a_cap)]];
             if (vals) vals[k] = v;
             ++k;
         }
-        if (rhs) rhs[r] = b[r];
-        if (ids) ids[r] = urow[r];
+        if (rhs) rhs[o] = b[r];
+        if (ids) ids[o] = urow[r];
     }
     if (rowptr) rowptr[nu] = (int32_t)k;
     return VOX_OK;
@@ -557,6 +648,7 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
     c->nu = nu;
     c->aval32_ok = false;
     c->aval64_ok = true;
+    c->uorder_cells = false;       // the rows keep the loaded order
     return VOX_OK;
 }
 
