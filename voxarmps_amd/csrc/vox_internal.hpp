@@ -108,8 +108,9 @@ struct DevList {
                                    // list_compact_ok); the reference rows are rebuilt on demand
     uint64_t epoch = 0;            // vox_ctx::pos_epoch when the rows were searched
     int32_t* pos_of = nullptr;     // tp: particle id -> its position in rec
-    bool pos_rows = false;         // row entries are positions in rec, not particle ids
-                                   // (list L's compact FP64 rows; see list_gather)
+    bool pos_rows = false;         // row entries are positions in rec, not particle ids,
+                                   // and each row (and count) sits at its particle's
+                                   // position too (list L's compact FP64 rows)
 };
 
 // Sliced row storage in chunks of 8 (SELL-64x8). Rows are grouped in slices

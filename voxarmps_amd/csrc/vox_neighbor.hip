@@ -459,7 +459,8 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
     const int ix = key % g.ncx;
     const int rowkey = key / g.ncx;                   // y + ncy z
     const long ncxy = (long)g.ncx * g.ncy;
-    const long rb = sl_index(i, 0, cap);
+    // rows of positions are stored at the particle's own position too
+    const long rb = sl_index(kPos ? t : (long)i, 0, cap);
     const double R2 = R * R;
     const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
     int k = 0;
@@ -520,7 +521,7 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
     if (valid && (k & 7)) flush_ids(ring, k >> 3, cap, ids + rb);
     int kmax = 0;
     if (valid) {
-        cnt[i] = k;
+        cnt[kPos ? t : (long)i] = k;
         if (k > cap) atomicAdd(&flags[flag_overflow], 1);
         kmax = k;
     }

@@ -181,13 +181,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
     const int self = a.pos_of ? a.pos_of[i] : i;   // own index among the entries' records
     int32_t* rc = ring_c + threadIdx.x * kRing;
     double* rv = ring_v + threadIdx.x * kRing;
-    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    // rows of positions sit at the particle's position
+    const int32_t* __restrict__ row = a.L.ids + sl_index(self, 0, a.L.cap);
     const PosRec me = a.rec[self];
     const double xi = me.x, yi = me.y, zi = me.z;
     const long ob = mx_index(r, 0, a.acap);                  // entry k at ob + 64 k
     double diag = 0.0;
     int k = 0, l0 = 0;
-    const int cnt = live ? a.L.cnt[i] : 0;
+    const int cnt = live ? a.L.cnt[self] : 0;
     constexpr int U = 8;
     for (int g = 0;; ++g) {
         // this lane's rows until group g is complete or the row is done; a

@@ -76,8 +76,8 @@ __global__ __launch_bounds__(kBlock) void k_pnd(int n, int ktype, double re, dou
     const PosRec me = rec[self];
     double sum = 0.0;
     const double inv_re = 1.0 / re;
-    const int cnt = L.cnt[i];
-    const int32_t* __restrict__ rowp = L.ids + sl_index(i, 0, L.cap);
+    const int cnt = L.cnt[self];                 // rows of positions sit at positions
+    const int32_t* __restrict__ rowp = L.ids + sl_index(self, 0, L.cap);
     for (int g = 0; (g << 3) < cnt; ++g) {
         int32_t jc[8];
         load_chunk8(rowp, g, jc);
@@ -485,8 +485,8 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
     const double inv_rhoN = 1.0 / rhoN;
     const double inv_re_s = 1.0 / a.re_s, inv_re_l = 1.0 / a.re_l;
     double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
-    const int cnt = a.L.cnt[i];
-    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    const int cnt = a.L.cnt[self];               // rows of positions sit at positions
+    const int32_t* __restrict__ row = a.L.ids + sl_index(self, 0, a.L.cap);
     // One row chunk (8 ids) per outer step, consumed in groups of R: a group's
     // record and type loads are issued together, then its terms are
     // accumulated in row order.
