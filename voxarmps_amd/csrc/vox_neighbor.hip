@@ -497,19 +497,24 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
 #pragma unroll
                             for (int uu = 0; uu < 8; ++uu) {
                                 const int u = u0 + uu;
-                                const PosRec cu = tile[u];
                                 // z is 0 in 2D records
-                                const double dx = cu.x - me.x, dy = cu.y - me.y, dz = cu.z - me.z;
+                                const double dx = tile[u].x - me.x, dy = tile[u].y - me.y,
+                                             dz = tile[u].z - me.z;
                                 const double d2f = fma(dz, dz, fma(dy, dy, dx * dx));
-                                bool acc = d2f < lo;
-                                if (!acc && !(d2f > hi)) {   // rare band: the reference's expression
+                                const bool sure = d2f < lo;
+                                const bool near = d2f <= hi;
+                                bool acc = sure;
+                                if (near != sure) {          // rare band: the reference's expression
                                     const double d2 = dx * dx + dy * dy + dz * dz;
                                     acc = g.incl ? (sqrt(d2) <= R) : (sqrt(d2) < R);
                                 }
-                                acc = acc && on && (u < nc) && (g.incl || cu.id != i);
+                                // the particle itself: by its position in rec (rows
+                                // of positions), else by id
+                                const bool other = kPos ? (j0 + u != t) : (tile[u].id != i);
+                                acc = acc && on && (u < nc) && (g.incl || other);
                                 // the next free slot takes every candidate; only
                                 // an accepted one advances past it
-                                ring[k & 15] = kPos ? j0 + u : cu.id;
+                                ring[k & 15] = kPos ? j0 + u : tile[u].id;
                                 k += acc ? 1 : 0;
                             }
                             if ((k0 >> 3) != (k >> 3)) flush_ids(ring, k0 >> 3, cap, ids + rb);
