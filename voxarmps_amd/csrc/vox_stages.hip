@@ -517,23 +517,29 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
         const double D = sqrt(dx * dx + dy * dy + dz * dz);
         const bool use = (jj[u] != self) && (D > 0);
         const double mj = rj.meu;
-        // every quotient below equals the plain IEEE division (div_by); the
-        // six divisions by D share one reciprocal
-        const double inv_D = 1.0 / D;
         const double NEU = (kUniform || ti == tj[u])
                                ? div_by(mi, rhoN, inv_rhoN)
                                : div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
-        const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
-        const double wl = (a.re_l == a.re_s) ? wg : wfun_r(D, a.ktype, a.re_l, inv_re_l);
         const double dp = rj.pn - phi_;
         const double du = rj.u - ui, dv = rj.v - vi, dw = rj.w - wi;
-        s1 = use ? s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D) : s1;
-        s2 = use ? s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D) : s2;
+        // The gradient terms carry W(D; re_s), which is 0 for D / re_s > 1
+        // (every kernel type): there each term is +-0 for finite fields, and
+        // adding it leaves the sum's bits unchanged (a sum that starts at +0
+        // is never -0), so it is skipped. On list L (radius re_l = 4 l0 against re_s = 2.1 l0)
+        // that is most entries. Every quotient equals the plain IEEE division
+        // (div_by); the divisions by D share one reciprocal.
+        if (use && div_by(D, a.re_s, inv_re_s) <= 1.0) {
+            const double inv_D = 1.0 / D;
+            const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
+            s1 = s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D);
+            s2 = s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D);
+            s3 = s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D);   // z sum: only read in 3D
+        }
+        const double wl = wfun_r(D, a.ktype, a.re_l, inv_re_l);
         s4 = use ? s4 + wl * du * NEU : s4;
         s5 = use ? s5 + wl * dv * NEU : s5;
         s7 = use ? s7 + wl * du : s7;
         s8 = use ? s8 + wl * dv : s8;
-        s3 = use ? s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D) : s3;   // z sums: only read in 3D
         s6 = use ? s6 + wl * dw * NEU : s6;
         s9 = use ? s9 + wl * dw : s9;
     }
