@@ -391,7 +391,6 @@ int build_neighbors(vox_ctx* c);
 bool list_compact_ok(const vox_ctx* c);
 int ensure_reference_list(vox_ctx* c, int which);
 int list_l_for_consumer(vox_ctx* c);
-int list_l_ids(vox_ctx* c);     // list L's rows as particle ids (consumers that need them)
 // The cell-sorted records of list `which` as a per-particle kernels' thread
 // order (sorted_particle), or null before the first NEIGHBOR.
 const PosRec* list_order(const vox_ctx* c, int which);

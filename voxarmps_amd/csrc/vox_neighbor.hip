@@ -677,10 +677,11 @@ static int search_one(vox_ctx* c, int which, double re)
     const bool cell_walk = (mode == 2 || (mode == 0 && per_cell >= 24.0));
     ProfScope ps(c, which == 0 ? KC_LIST : KC_LIST_L);
     VOX_TRY(fill_i32(c, c->dflags + fo, 0, 1));
-    // Compact FP64 rows of the cell walk hold positions in the cell-sorted
-    // records: their consumers gather from records in that order, where a
-    // row's consecutive entries (one cell's particles) sit side by side.
-    L.pos_rows = compact && cell_walk && c->p.precision == VOX_PREC_FP64;
+    // Compact rows of the cell walk hold positions in the cell-sorted records
+    // (and sit at the particle's position): their consumers gather from
+    // records in that order, where a row's consecutive entries (one cell's
+    // particles) sit side by side.
+    L.pos_rows = compact && cell_walk;
     if (cell_walk) {
         VOX_TRY(fill_i32(c, c->dflags + fm, 0, 1));
         if (L.pos_rows)
@@ -765,14 +766,6 @@ int ensure_reference_list(vox_ctx* c, int which)
         VOX_TRY(ensure_list_storage(c, which, ((need + need / 4) + 7) & ~7));
     }
     return fail(VOX_ERR_OVERFLOW, "neighbour row capacity could not be satisfied");
-}
-
-// List L's rows as particle ids: rows of positions are replaced by the
-// reference rows (searched over the same records).
-int list_l_ids(vox_ctx* c)
-{
-    if (c->two_lists && c->list[1].pos_rows) return ensure_reference_list(c, 1);
-    return VOX_OK;
 }
 
 // Called by every consumer of list L before it reads the rows: compact rows

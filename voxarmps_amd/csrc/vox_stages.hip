@@ -610,10 +610,7 @@ const char* pred_variant_name(int v)
 int st_prediction(vox_ctx* c)
 {
     VOX_TRY(list_l_for_consumer(c));
-    if (c->p.precision == VOX_PREC_MIXED) {
-        VOX_TRY(list_l_ids(c));           // the FP32 predictor reads particle ids
-        return st_prediction32(c);
-    }
+    if (c->p.precision == VOX_PREC_MIXED) return st_prediction32(c);
     ProfScope ps(c, KC_PRED);
     const vox_params& p = c->p;
     PredArgs a;

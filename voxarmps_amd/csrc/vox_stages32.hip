@@ -83,20 +83,27 @@ struct Pred32Args {
     const int32_t* pt;
     const double *x, *y, *z, *u, *v, *w, *pn, *ph, *meu, *neut, *rho, *t1, *t2, *t3;
     double *xs, *ys, *zs, *us, *vs, *ws;
-    Pred32* rec;
+    Pred32* rec;              // the records the row entries index (see pos)
+    const PosRec* order;      // thread t takes particle order[t].id (cell-sorted)
+    int32_t* rpt;             // pos: the types in the same order as rec
+    bool pos;                 // row entries are positions in order (list L's compact
+                              // rows, stored at positions): rec and rpt are in that
+                              // order and thread t's own record is rec[t]
 };
 
 __global__ void k_pack_pred32(Pred32Args a)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int i = a.pos ? a.order[t].id : t;
+    if (a.pos) a.rpt[t] = a.pt[i];
     Pred32 r;
     r.x = qpos(a.x[i], a.q.ox, a.q.ix);
     r.y = qpos(a.y[i], a.q.oy, a.q.iy);
     r.z = (a.dim == 3) ? qpos(a.z[i], a.q.oz, a.q.iz) : 0u;
     r.u = (float)a.u[i]; r.v = (float)a.v[i]; r.w = (a.dim == 3) ? (float)a.w[i] : 0.0f;
     r.pn = (float)a.pn[i]; r.meu = (float)a.meu[i];
-    a.rec[i] = r;
+    a.rec[t] = r;
 }
 
 // kUniform: every MEU is the same value (see uniform_meu): NEU does not
@@ -104,8 +111,11 @@ __global__ void k_pack_pred32(Pred32Args a)
 template <bool kUniform>
 __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
 {
-    const int i = xcd_tid();
-    if (i >= a.n) return;
+    const int t = xcd_tid();
+    if (t >= a.n) return;
+    const int i = sorted_particle(a.order, t);
+    const int self = a.pos ? t : i;          // own index among the row entries' records
+    const int32_t* __restrict__ rpt = a.pos ? a.rpt : a.pt;
     const int dim = a.dim;
     const int ti = a.pt[i];
     if (ti <= 0) {
@@ -113,23 +123,23 @@ __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
         if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
         return;
     }
-    const Pred32 me = a.rec[i];
+    const Pred32 me = a.rec[self];
     const float mi = me.meu, phi_ = (float)a.ph[i];
     const float rhoN = (ti == 1) ? a.rho1 : a.rho2;
     float s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
-    const int cnt = a.L.cnt[i];
-    const int32_t* __restrict__ row = a.L.ids + sl_index(i, 0, a.L.cap);
+    const int cnt = a.L.cnt[self];              // rows of positions sit at positions
+    const int32_t* __restrict__ row = a.L.ids + sl_index(self, 0, a.L.cap);
     constexpr int U = 8;
     for (int l0 = 0; l0 < cnt; l0 += U) {
         int jj[U], tj[U];
         Pred32 rv[U];
         load_chunk8(row, l0 >> 3, jj);
 #pragma unroll
-        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : i;
+        for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : self;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             rv[u] = a.rec[jj[u]];
-            tj[u] = kUniform ? ti : a.pt[jj[u]];
+            tj[u] = kUniform ? ti : rpt[jj[u]];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -137,17 +147,19 @@ __global__ __launch_bounds__(kBlock) void k_prediction32(Pred32Args a)
             const float dx = qdiff(rj.x, me.x, a.q.sx), dy = qdiff(rj.y, me.y, a.q.sy),
                         dz = qdiff(rj.z, me.z, a.q.sz);
             const float D = sqrtf(dx * dx + dy * dy + dz * dz);
-            const bool use = (jj[u] != i) && (D > 0.0f);
+            const bool use = (jj[u] != self) && (D > 0.0f);
             const float NEU = (kUniform || ti == tj[u]) ? mi / rhoN
                                                         : 2.0f * mi * rj.meu / (mi + rj.meu) / rhoN;
-            const float wg = wfun32(D, a.ktype, a.re_s);
-            const float wl = (a.re_l == a.re_s) ? wg : wfun32(D, a.ktype, a.re_l);
+            const float wl = wfun32(D, a.ktype, a.re_l);
             const float dp = rj.pn - phi_;
             const float du = rj.u - me.u, dv = rj.v - me.v, dw = rj.w - me.w;
-            const float g = wg / D / D;
-            s1 = use ? s1 + dp * dx * g : s1;
-            s2 = use ? s2 + dp * dy * g : s2;
-            s3 = use ? s3 + dp * dz * g : s3;
+            // the gradient terms carry W(D; re_s) = 0 beyond re_s: skipped there
+            if (use && D <= a.re_s) {
+                const float g = wfun32(D, a.ktype, a.re_s) / D / D;
+                s1 = s1 + dp * dx * g;
+                s2 = s2 + dp * dy * g;
+                s3 = s3 + dp * dz * g;
+            }
             s4 = use ? s4 + wl * du * NEU : s4;
             s5 = use ? s5 + wl * dv * NEU : s5;
             s6 = use ? s6 + wl * dw * NEU : s6;
@@ -197,6 +209,10 @@ int st_prediction32(vox_ctx* c)
     a.xs = c->f[VOX_F_XSTAR]; a.ys = c->f[VOX_F_YSTAR]; a.zs = c->f[VOX_F_ZSTAR];
     a.us = c->f[VOX_F_USTAR]; a.vs = c->f[VOX_F_VSTAR]; a.ws = c->f[VOX_F_WSTAR];
     a.rec = reinterpret_cast<Pred32*>(c->pred_rec);          // 64-byte slots hold 32-byte records
+    // threads in list L's cell order; rows of positions index records in that order
+    a.order = list_order(c, VOX_LIST_L);
+    a.pos = list_of(c, VOX_LIST_L).pos_rows && a.order;
+    a.rpt = c->pt_sorted;
     k_pack_pred32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     if (uniform_meu(c)) k_prediction32<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     else k_prediction32<false><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
