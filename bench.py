@@ -37,10 +37,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_cg_spmv_dam3d_1175k.json")
 
 
-def measured_traffic(precision, nu, nnz):
+def measured_traffic(precision, nu, nnz, cbytes):
     """HBM bytes per k_cg_spmv launch from the committed counter summary, or
     None when no summary matches this workload (same unknown and nonzero
-    counts within 1%)."""
+    counts within 1%, same column format)."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -48,6 +48,8 @@ def measured_traffic(precision, nu, nnz):
         return None
     e = t.get(precision)
     if not e or abs(e["unknowns"] - nu) > 0.01 * nu or abs(e["nnz"] - nnz) > 0.01 * nnz:
+        return None
+    if e.get("column_bytes", 4) != cbytes:
         return None
     return e["bytes_per_launch"]
 
@@ -61,12 +63,14 @@ def list_counts(sim, which):
     return cnt
 
 
-def step_bytes(sim, case, nu, nnz_total, iters, s=8):
+def step_bytes(sim, case, nu, nnz_total, iters, s=8, cb=4, wc=0):
     """Compulsory HBM bytes of one time step, stage by stage: every array read
     once and written once, neighbour-gathered fields counted once, the index
     list of the radius used read once per consuming stage (4 bytes an entry
     plus its count). Lists, unknowns, nonzeros and CG iterations are those of
-    the run. FP64 values (s = 8)."""
+    the run. FP64 values (s = 8); cb bytes per PPE column the product reads
+    (2: 16-bit steps and a 4-byte first column per row, or 16-bit ranks in
+    LDS windows whose wc columns in all take 4 bytes each)."""
     import numpy as np
     import voxarmps_amd as vx
     N, d = case.tp, case.dim
@@ -78,6 +82,7 @@ def step_bytes(sim, case, nu, nnz_total, iters, s=8):
     S, Lsum = int(cs.sum()), int(cl.sum())
     SF, LF, LU = int(cs[fl].sum()), int(cl[fl].sum()), int(cl[bc == 0].sum())
     off = nnz_total - nu
+    rb = 4 if cb == 2 and not wc else 0      # first column per row
     parts = {
         "neighbour lists": 2 * N * d * 8 + 4 * (S + Lsum),
         "pnd (n, n*)": 2 * (4 * S + N * (d * s + s)),
@@ -86,8 +91,8 @@ def step_bytes(sim, case, nu, nnz_total, iters, s=8):
         "viscosity": N * (4 + s),
         "predictor": 4 * LF + N * (2 * d * s + 2 * s + 4) + F * (3 * s + 2 * d * s),
         "bcon": N * (8 + s),
-        "assembly": 4 * LU + N * (d * s + 4) + nu * 2 * s + off * (4 + s) + 4 * (nu + 1),
-        "cg": iters * (off * (4 + s) + 4 * (nu + 1) + 13 * s * nu),
+        "assembly": 4 * LU + N * (d * s + 4) + nu * 2 * s + off * (cb + s) + (4 + rb) * nu + 4 + 4 * wc,
+        "cg": iters * (off * (cb + s) + (4 + rb) * nu + 4 + 4 * wc + 13 * s * nu),
         "phat": 4 * S + 2 * N * s,
         "gradient": 4 * SF + N * (d * s + 2 * s) + F * (2 * d * s + 2 * s),
         "euler": 4 * F * d * s,
@@ -97,14 +102,17 @@ def step_bytes(sim, case, nu, nnz_total, iters, s=8):
     return sum(parts.values()), parts
 
 
-def spmv_bytes(nu, nnz_total, vbytes=8):
+def spmv_bytes(nu, nnz_total, vbytes=8, cbytes=4, window_cols=0):
     """Compulsory bytes of one q = A p launch: off-diagonal values (vbytes)
-    and column ids (4 B), diagonal (8 B, FP64 in both modes), row length
-    (4 B), p read once and q written (vbytes each) per row; gathers of p are
-    counted once. vbytes is 8 in FP64 and 4 for the FP32 inner solves of the
-    mixed mode."""
+    and columns (cbytes: 4 for 32-bit ids; 2 for 16-bit steps plus the
+    first column of each row, 4 B, or for 16-bit ranks in the blocks' LDS
+    windows plus each window's column list, 4 B per column), diagonal (8 B,
+    FP64 in both modes), row length (4 B), p read once and q written
+    (vbytes each) per row; gathers of p are counted once. vbytes is 8 in
+    FP64 and 4 for the FP32 inner solves of the mixed mode."""
     off = nnz_total - nu
-    return off * (4 + vbytes) + nu * (8 + 4 + 2 * vbytes)
+    per_row = 8 + 4 + 2 * vbytes + (4 if cbytes == 2 and not window_cols else 0)
+    return off * (cbytes + vbytes) + nu * per_row + 4 * window_cols
 
 
 def cpu_baseline(sample_scale=2, min_s=10.0, max_steps=20):
@@ -501,9 +509,10 @@ def main():
     nu = int(info["rowptr"].shape[0] - 1)
     nnz = int(info["rowptr"][-1])
     del info
+    cbytes = st.product_col_bytes or 4
     step_total, step_parts = (None, None) if (ensemble or mixed) else \
-        step_bytes(sim, case, nu, nnz, sum(iters) / len(iters))
-    bytes_launch = spmv_bytes(nu, nnz, 4 if mixed else 8)
+        step_bytes(sim, case, nu, nnz, sum(iters) / len(iters), cb=cbytes, wc=st.product_window_cols)
+    bytes_launch = spmv_bytes(nu, nnz, 4 if mixed else 8, cbytes, st.product_window_cols)
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     per_kernel = {}
     for k in ("list", "list_l", "cell", "sort", "pnd", "sps", "pred", "assemble", "spmv", "cg_update", "cg_p",
@@ -585,8 +594,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-            "traffic": measured_traffic("mixed" if mixed else "fp64", nu, nnz),
+            "traffic": measured_traffic("mixed" if mixed else "fp64", nu, nnz, cbytes),
             "bytes_per_launch": bytes_launch,
+            "column_bytes": cbytes,
+            "window_columns": st.product_window_cols,
             "avg_launch_ms": avg_ms,
             "launches": nl.value,
             "unknowns": nu,

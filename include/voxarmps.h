@@ -166,6 +166,10 @@ typedef struct vox_status {
     double  time;             /* simulated time                                  */
     int32_t lpc_fallbacks;    /* learned solves that fell back to Jacobi-CG      */
     int32_t lpc_baseline;     /* Jacobi-CG iterations the learned solver is held to */
+    int32_t product_col_bytes;/* bytes per column the last PPE product read: 2
+                                 (16-bit steps or window ranks) or 4           */
+    int32_t product_window_cols; /* columns the last product staged in its LDS
+                                 windows, summed over its blocks (0: none)     */
 } vox_status;
 
 typedef struct vox_ctx vox_ctx;
@@ -281,12 +285,16 @@ int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const doub
  * list (modes[2] receives the choice for lists S and L); then runs the step's
  * stages up to MATRIX once and times the predictor's variants (FP64: records
  * gathered per group 4, 8 or 2 with 256 threads per block, 4 or 8 with 128)
- * and the PPE product's (8, 4 or 16 entries per load batch). Every variant
- * gives the same bits; the context keeps the fastest. Requires vox_init. */
+ * and the PPE product's (0: 8 entries per load batch with p gathered from a
+ * per-block LDS window of the block's distinct columns; 1, 2, 3: 4, 16 and 8
+ * entries on the 32-bit columns; 4: 8 entries on 16-bit column steps; 0 and 4
+ * fall back to the next form where the assembly's rows do not allow theirs).
+ * Every variant gives the same bits; the context keeps the fastest. Requires
+ * vox_init. */
 int vox_autotune(vox_ctx* ctx, int reps, int32_t* modes);
 /* The context's tuning (no counterpart in the reference): choice[4] = walk of
  * list S, walk of list L (0 = heuristic), predictor variant (0..4), product
- * variant (0..2); vox_set_tuning applies a cached choice without timing. */
+ * variant (0..4); vox_set_tuning applies a cached choice without timing. */
 int vox_get_tuning(vox_ctx* ctx, int32_t* choice);
 int vox_set_tuning(vox_ctx* ctx, const int32_t* choice);
 

@@ -6,7 +6,9 @@
   initialisation of main.cu:966-1012) matches the oracle restarted from the
   same file.
 * vox_ppe_load is the inverse of vox_ppe_export: a loaded system exports back
-  unchanged, and vox_ppe_solve solves it (residual recomputed in numpy)."""
+  unchanged, and vox_ppe_solve solves it (residual recomputed in numpy),
+  through the product on 32-bit columns (an assembled system's product
+  gathers from LDS windows)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -83,6 +85,10 @@ def test_ppe_load_is_the_inverse_of_export(hip, dim):
     A = s.ppe()
     nu = A["rhs"].shape[0]
     assert nu > 50
+    # the assembled rows ascend (unknowns in the list's cell order): the
+    # product gathers p from its blocks' LDS windows by 16-bit ranks
+    s.CGM()
+    assert s.status().product_col_bytes == 2 and s.status().product_window_cols > 0
     # a second context of the same case, never assembled: load, export, solve
     t, _, _ = _run(hip, dim, 0)
     rng = np.random.default_rng(dim)
@@ -93,6 +99,8 @@ def test_ppe_load_is_the_inverse_of_export(hip, dim):
     for k in ("rowptr", "cols", "vals", "rhs", "ids"):
         assert np.array_equal(E[k], B[k]), k
     info = t.CGM()
+    st = t.status()                                   # loaded rows: 32-bit columns
+    assert st.product_col_bytes == 4 and st.product_window_cols == 0
     x = t.get("PNEW")
     M = sp.csr_matrix((A["vals"], A["cols"], A["rowptr"]), shape=(nu, nu))
     xu = x[A["ids"]]

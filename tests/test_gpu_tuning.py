@@ -1,7 +1,8 @@
 """The autotuner's kernel variants give the same bits: every predictor
 variant (records gathered per group x threads per block) and every PPE
-product variant (entries per load batch) against the default on the same
-state, and a tuned run against an untuned one."""
+product variant (entries per load batch; columns as 16-bit steps or 32-bit
+ids) against the default on the same state, and a tuned run against an
+untuned one."""
 import numpy as np
 import pytest
 
@@ -38,15 +39,23 @@ def test_kernel_variants_are_bit_identical(hip, precision):
     for st in ("PNUMSTAR", "BCON", "MATRIX"):
         getattr(s, st)()
     ref = None
-    for v in range(3):
+    for v in range(5):
         s.set_tuning([0, 0, 0, v])
         info = s.CGM()
         out = (s.get("PNEW"), info.iterations)
+        # the unknowns are numbered in the list's cell order, so every row's
+        # columns ascend: variant 0 reads them as ranks in its blocks' LDS
+        # windows, variant 4 as 16-bit steps
+        st = s.status()
+        assert st.product_col_bytes == (2 if v in (0, 4) else 4), v
+        assert (st.product_window_cols > 0) == (v == 0), v
         if ref is None:
             ref = out
         assert np.array_equal(out[0], ref[0]) and out[1] == ref[1], f"product variant {v}"
     with pytest.raises(vx.VoxError):
         s.set_tuning([0, 0, 5, 0])
+    with pytest.raises(vx.VoxError):
+        s.set_tuning([0, 0, 0, 5])
     s.close()
 
 
@@ -60,7 +69,7 @@ def test_autotuned_run_matches_untuned(hip):
     b.autotune(reps=2)
     t = b.tuning()
     print("tuning (walk S, walk L, predictor, product):", t)
-    assert t[0] in (1, 2) and t[1] in (1, 2) and 0 <= t[2] < 5 and 0 <= t[3] < 3
+    assert t[0] in (1, 2) and t[1] in (1, 2) and 0 <= t[2] < 5 and 0 <= t[3] < 5
     a.step(10)
     b.step(10)
     for n in ("X", "Y", "Z", "U", "V", "W", "P"):

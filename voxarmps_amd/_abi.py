@@ -81,7 +81,8 @@ class VoxStatus(C.Structure):
         ("nan_count", _i32), ("out_of_box", _i32), ("max_count_s", _i32), ("max_count_l", _i32),
         ("collisions", _i32), ("collision_cand", _i32), ("steps", _i32), ("last_iters", _i32),
         ("last_relres", _f64), ("n0_s", _f64), ("n0_l", _f64), ("time", _f64),
-        ("lpc_fallbacks", _i32), ("lpc_baseline", _i32),
+        ("lpc_fallbacks", _i32), ("lpc_baseline", _i32), ("product_col_bytes", _i32),
+        ("product_window_cols", _i32),
     ]
 
 

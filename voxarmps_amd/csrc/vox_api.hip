@@ -487,7 +487,7 @@ int vox_destroy(vox_ctx* c)
     }
     for (int k = 0; k < 6; ++k) dfree(c->sps_s[k]);
     dfree(c->scratch_d); dfree(c->pred_rec); dfree(c->pt_sorted); dfree(c->pos_rec); dfree(c->blockmax); dfree(c->blockmax2);
-    dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->uorder); dfree(c->uscan); dfree(c->zcell); dfree(c->rowlen); dfree(c->acol);
+    dfree(c->uidx); dfree(c->urow); dfree(c->uflag); dfree(c->uorder); dfree(c->uscan); dfree(c->zcell); dfree(c->rowlen); dfree(c->acol); dfree(c->dcol); dfree(c->cbase); dfree(c->wcol); dfree(c->wuniq); dfree(c->wn);
     dfree(c->aval); dfree(c->adiag); dfree(c->rhs);
     dfree(c->cg_x); dfree(c->cg_r); dfree(c->cg_p); dfree(c->cg_q); dfree(c->cg_d); dfree(c->cg_e);
     dfree(c->cg_x32); dfree(c->cg_r32); dfree(c->cg_p32); dfree(c->cg_q32); dfree(c->aval32);

@@ -54,6 +54,12 @@ struct CgArgs {
     double tol;
     const int32_t* __restrict__ rowlen;
     const int32_t* __restrict__ acol;
+    const v4u* __restrict__ dcol;     // the columns as 16-bit steps (null: not valid)
+    const int32_t* __restrict__ cbase;
+    const v4u* __restrict__ wcol;     // ranks in the chunk windows (null: not built)
+    const int32_t* __restrict__ wuniq;
+    const int32_t* __restrict__ wn;
+    int win_n;
     const MT* __restrict__ aval;
     const double* __restrict__ diag;
     const double* __restrict__ rhs;   // FP64 right-hand side of this solve
@@ -136,15 +142,57 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
 
 // q = A p and the partial p.q of each block; one thread per row. U: entries
 // whose loads are issued together (the sum is in row order for every U, so
-// the autotuner's choice does not change any bit).
-template <class VT, class MT, int U = 8>
+// the autotuner's choice does not change any bit). kD: the columns come from
+// their 16-bit steps (U = 8: one 16-byte chunk per lane per batch, 10 instead
+// of 12 bytes per FP64 entry and one column load per 8 entries, not 8).
+template <class VT, class MT, int U = 8, bool kD = false>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
 {
+    static_assert(!kD || U == 8, "column steps come in chunks of 8");
     __shared__ double lds[kBlock];
     if (a.flags[FLAG_CG_DONE]) return;
     double pq = 0.0;
     const int r = xcd_tid();
-    if (r < a.nu) {
+    if (kD && r < a.nu) {
+        const VT pr = a.p[r];
+        VT acc = (VT)a.diag[r] * pr;
+        const int len = a.rowlen[r];
+        const v4u* __restrict__ dc = a.dcol + dc_index(r, 0, a.cap);   // chunk g at dc[64 g]
+        const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
+        int cj = a.cbase[r];
+        int k = 0;
+        for (; k + 8 <= len; k += 8) {
+            const v4u w = __builtin_nontemporal_load(&dc[8L * k]);
+            MT vv[8];
+            VT pp[8];
+            int cc[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = __builtin_nontemporal_load(&val[64L * (k + u)]);
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                cj += (int)(w[h] & 0xffffu);
+                cc[2 * h] = cj;
+                cj += (int)(w[h] >> 16);
+                cc[2 * h + 1] = cj;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pp[u] = a.p[cc[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + (VT)vv[u] * pp[u];
+        }
+        if (k < len) {
+            const v4u w = __builtin_nontemporal_load(&dc[8L * k]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (k + u < len) {
+                    cj += (int)((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+                    acc = acc + (VT)val[64L * (k + u)] * a.p[cj];
+                }
+            }
+        }
+        a.q[r] = acc;
+        pq = (double)pr * (double)acc;
+    } else if (!kD && r < a.nu) {
         // (all 16 streaming loads of a step are issued before the first
         // gather: keep this loop's shape, other forms scheduled the gathers
         // earlier and measured about 18% slower; 8 entries per step measured
@@ -175,6 +223,219 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(CgArgs<VT, MT> a)
     }
     pq = block_sum(pq, lds);
     if (threadIdx.x == 0) PSLOT(a, P_PQ)[blockIdx.x] = pq;
+}
+
+// Step or rank u (0..7) of a 16-byte chunk of 16-bit values.
+__device__ __forceinline__ int u16_of(const v4u& w, int u)
+{
+    return (int)((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+}
+
+// The same product with p gathered from LDS: each block (one chunk of 256
+// rows, as in k_cg_spmv) first stages p at the chunk's distinct columns
+// (wuniq, ascending, so mostly whole lines) into a window in LDS, then reads
+// each entry's value and its 16-bit rank in that window. The gathers from
+// global memory, one cache-line lookup per distinct line touched by each
+// wave-wide load, become one coalesced staging pass per block; the row sums
+// are the same operations in the same order, so q and p.q are the same bits.
+template <class VT, class MT>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_win(CgArgs<VT, MT> a)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    VT* win = reinterpret_cast<VT*>(dyn_lds);
+    __shared__ double lds[kBlock];
+    if (a.flags[FLAG_CG_DONE]) return;
+    const int ch = xcd_block();
+    const int n = a.wn[ch];
+    const int32_t* __restrict__ uq = a.wuniq + (long)ch * kWinMax;
+    for (int t = threadIdx.x; t < n; t += kBlock) win[t] = a.p[uq[t]];
+    __syncthreads();
+    double pq = 0.0;
+    const int r = ch * kBlock + threadIdx.x;
+    if (r < a.nu) {
+        const VT pr = a.p[r];
+        VT acc = (VT)a.diag[r] * pr;
+        const int len = a.rowlen[r];
+        const v4u* __restrict__ wc = a.wcol + dc_index(r, 0, a.cap);
+        const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
+        int k = 0;
+        for (; k + 8 <= len; k += 8) {
+            const v4u w = __builtin_nontemporal_load(&wc[8L * k]);
+            MT vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = __builtin_nontemporal_load(&val[64L * (k + u)]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + (VT)vv[u] * win[u16_of(w, u)];
+        }
+        if (k < len) {
+            const v4u w = __builtin_nontemporal_load(&wc[8L * k]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < len) acc = acc + (VT)val[64L * (k + u)] * win[u16_of(w, u)];
+        }
+        a.q[r] = acc;
+        pq = (double)pr * (double)acc;
+    }
+    pq = block_sum(pq, lds);
+    if (threadIdx.x == 0) PSLOT(a, P_PQ)[blockIdx.x] = pq;
+}
+
+struct WinArgs {
+    int nu, cap, words;        // words: bitmap words in the dynamic LDS
+    const int32_t* __restrict__ rowlen;
+    const int32_t* __restrict__ acol;
+    const v4u* __restrict__ dcol;
+    const int32_t* __restrict__ cbase;
+    v4u* wcol;
+    int32_t* wuniq;
+    int32_t* wn;
+    int32_t* flags;
+};
+
+// The windows of the LDS-gather product, one block per chunk of 256 rows
+// (rows ascending, so the chunk's columns span [smallest first column,
+// largest last column]): a bitmap of the columns the rows use (in dynamic
+// LDS, `words` words, sized from the assembly's bound on the rows' reach),
+// the prefix of its popcounts per word, the distinct columns in ascending
+// order and each entry's rank among them. A chunk whose span exceeds the
+// bitmap or whose distinct columns exceed kWinMax raises FLAG_WIN_N past
+// kWinMax, and the product then reads the columns themselves. Each lane walks
+// its row 4 chunks (32 entries) per round trip.
+__global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
+{
+    extern __shared__ __align__(16) uint32_t wlds[];
+    uint32_t* bits = wlds;
+    uint16_t* pre = reinterpret_cast<uint16_t*>(wlds + a.words);
+    __shared__ int red[kBlock];
+    const int ch = blockIdx.x, t = threadIdx.x;
+    const int r = ch * kBlock + t;
+    const int len = (r < a.nu) ? a.rowlen[r] : 0;
+    const int first = (len > 0) ? a.cbase[r] : 0x7fffffff;
+    const int lastc = (len > 0) ? a.acol[mx_index(r, len - 1, a.cap)] : -1;
+    red[t] = first;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = min(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const int lo = red[0];
+    __syncthreads();
+    red[t] = lastc;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] = max(red[t], red[t + s]);
+        __syncthreads();
+    }
+    const int hi = red[0];
+    __syncthreads();
+    if (hi < lo) {                                     // no entries (uniform)
+        if (t == 0) a.wn[ch] = 0;
+        return;
+    }
+    const long span = (long)hi - lo + 1;
+    if (span > 32L * a.words) {                        // uniform
+        if (t == 0) { a.wn[ch] = -1; atomicMax(&a.flags[FLAG_WIN_N], kWinMax + 1); }
+        return;
+    }
+    const int nw = (int)((span + 31) >> 5);
+    for (int w = t; w < nw; w += kBlock) bits[w] = 0u;
+    __syncthreads();
+    const v4u* __restrict__ dc = a.dcol + dc_index(r, 0, a.cap);
+    int cj = first;
+    // a row's columns ascend: the bits of consecutive entries in one word are
+    // gathered in a register and set with one atomic when the word changes
+    int cw = -1;
+    uint32_t cm = 0u;
+    for (int k = 0; k < len; k += 32) {
+        v4u w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            w[q] = (k + 8 * q < len) ? dc[8L * (k + 8 * q)] : v4u{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (k + 8 * q + u < len) {
+                    cj += u16_of(w[q], u);
+                    const int o = cj - lo;
+                    if ((o >> 5) != cw) {
+                        if (cw >= 0) atomicOr(&bits[cw], cm);
+                        cw = o >> 5;
+                        cm = 0u;
+                    }
+                    cm |= 1u << (o & 31);
+                }
+            }
+        }
+    }
+    if (cw >= 0) atomicOr(&bits[cw], cm);
+    __syncthreads();
+    // exclusive prefix of the words' popcounts: thread t owns words [w0, w1)
+    const int per = (nw + kBlock - 1) / kBlock;
+    const int w0 = min(nw, t * per), w1 = min(nw, w0 + per);
+    int sum = 0;
+    for (int w = w0; w < w1; ++w) sum += __popc(bits[w]);
+    red[t] = sum;
+    __syncthreads();
+    for (int s = 1; s < kBlock; s <<= 1) {
+        const int v = (t >= s) ? red[t - s] : 0;
+        __syncthreads();
+        red[t] += v;
+        __syncthreads();
+    }
+    const int total = red[kBlock - 1];
+    if (total > kWinMax) {                             // uniform
+        if (t == 0) { a.wn[ch] = -1; atomicMax(&a.flags[FLAG_WIN_N], kWinMax + 1); }
+        return;
+    }
+    int run = red[t] - sum;
+    int32_t* __restrict__ uq = a.wuniq + (long)ch * kWinMax;
+    for (int w = w0; w < w1; ++w) {
+        pre[w] = (uint16_t)run;
+        uint32_t b = bits[w];
+        while (b) {
+            uq[run++] = lo + 32 * w + (__ffs(b) - 1);
+            b &= b - 1u;
+        }
+    }
+    __syncthreads();
+    v4u* __restrict__ wc = a.wcol + dc_index(r, 0, a.cap);
+    cj = first;
+    cw = -1;
+    uint32_t cb = 0u;                                  // bits of word cw
+    int cp = 0;                                        // and its prefix
+    for (int k = 0; k < len; k += 32) {
+        v4u w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            w[q] = (k + 8 * q < len) ? dc[8L * (k + 8 * q)] : v4u{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (k + 8 * q < len) {
+                uint32_t o4[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (k + 8 * q + u < len) {
+                        cj += u16_of(w[q], u);
+                        const int o = cj - lo;
+                        if ((o >> 5) != cw) {
+                            cw = o >> 5;
+                            cb = bits[cw];
+                            cp = pre[cw];
+                        }
+                        const int rank = cp + __popc(cb & ((1u << (o & 31)) - 1u));
+                        o4[u >> 1] |= (uint32_t)rank << (16 * (u & 1));
+                    }
+                }
+                wc[8L * (k + 8 * q)] = v4u{o4[0], o4[1], o4[2], o4[3]};
+            }
+        }
+    }
+    if (t == 0) {
+        a.wn[ch] = total;
+        atomicMax(&a.flags[FLAG_WIN_N], total);
+        atomicAdd(&a.flags[FLAG_WIN_SUM], total);
+    }
 }
 
 template <class VT, class MT>
@@ -505,10 +766,27 @@ __global__ void k_scatter_x(int nu, const int32_t* __restrict__ urow, const doub
 }
 
 // ---- host ------------------------------------------------------------------------------------
-// The product variant the autotuner chose (8 entries per batch unless tuned).
+// The product variant the autotuner chose (variant 0 unless tuned). Variant 0
+// gathers p from the blocks' LDS windows when they are built, variant 4 reads
+// the columns' 16-bit steps when the assembly stored them all; either falls
+// back to the next form (window -> steps -> 32-bit columns). Variants 1-3
+// read the 32-bit columns, 4, 16 and 8 entries per batch.
 template <class VT, class MT>
 static void launch_spmv(vox_ctx* c, const CgArgs<VT, MT>& a, int variant)
 {
+    if (variant == 0 && a.wcol) {
+        c->st.product_col_bytes = 2;
+        c->st.product_window_cols = c->win_sum;
+        k_cg_spmv_win<VT, MT><<<a.npq, kBlock, sizeof(VT) * (size_t)a.win_n, c->stream>>>(a);
+        return;
+    }
+    const bool steps = (variant == 0 || variant == 4) && a.dcol != nullptr;
+    c->st.product_col_bytes = steps ? 2 : 4;
+    c->st.product_window_cols = 0;
+    if (steps) {
+        k_cg_spmv<VT, MT, 8, true><<<a.npq, kBlock, 0, c->stream>>>(a);
+        return;
+    }
     switch (variant) {
     case 1: k_cg_spmv<VT, MT, 4><<<a.npq, kBlock, 0, c->stream>>>(a); break;
     case 2: k_cg_spmv<VT, MT, 16><<<a.npq, kBlock, 0, c->stream>>>(a); break;
@@ -620,13 +898,73 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
     return VOX_OK;
 }
 
+// Builds the LDS windows of the current assembly once (k_win_build); win_ok
+// says whether every chunk fit.
+static int ensure_window(vox_ctx* c)
+{
+    if (c->win_asm == c->asm_count) return VOX_OK;
+    c->win_asm = c->asm_count;
+    c->win_ok = false;
+    if (!c->dcol_ok || c->nu <= 0) return VOX_OK;
+    const long nch = grid_for(c->nu);
+    if (c->wcap_alloc != c->a_cap) {
+        if (c->wcol) (void)hipFree(c->wcol);
+        c->wcol = nullptr;
+        c->wcap_alloc = 0;
+        VOX_HIP(hipMalloc(&c->wcol, sizeof(uint16_t) * (size_t)sl_size(c->tp, c->a_cap)));
+        c->wcap_alloc = c->a_cap;
+    }
+    if (c->wchunks_alloc < nch) {
+        if (c->wuniq) (void)hipFree(c->wuniq);
+        if (c->wn) (void)hipFree(c->wn);
+        c->wuniq = nullptr;
+        c->wn = nullptr;
+        c->wchunks_alloc = 0;
+        const long m = grid_for(c->tp);
+        VOX_HIP(hipMalloc(&c->wuniq, sizeof(int32_t) * (size_t)m * kWinMax));
+        VOX_HIP(hipMalloc(&c->wn, sizeof(int32_t) * (size_t)m));
+        c->wchunks_alloc = m;
+    }
+    // a chunk's columns lie in [r0 - below, r0 + 255 + above]
+    const long span = (long)c->col_below + c->col_above + kBlock;
+    const int words = (int)((span + 31) / 32);
+    if (words > kWinSpanWords) return VOX_OK;
+    VOX_TRY(fill_i32(c, c->dflags + FLAG_WIN_N, 0, 2));          // and FLAG_WIN_SUM
+    WinArgs w;
+    w.nu = c->nu; w.cap = c->a_cap; w.words = words; w.rowlen = c->rowlen; w.acol = c->acol;
+    w.dcol = reinterpret_cast<const v4u*>(c->dcol); w.cbase = c->cbase;
+    w.wcol = reinterpret_cast<v4u*>(c->wcol); w.wuniq = c->wuniq; w.wn = c->wn; w.flags = c->dflags;
+    {
+        ProfScope ps(c, KC_ASSEMBLE);
+        const size_t lds = sizeof(uint32_t) * (size_t)words + sizeof(uint16_t) * (size_t)(words + 1);
+        k_win_build<<<(unsigned)nch, kBlock, lds, c->stream>>>(w);
+        VOX_HIP(hipGetLastError());
+    }
+    VOX_TRY(read_flags(c));
+    c->win_n = c->hflags[FLAG_WIN_N];
+    c->win_sum = c->hflags[FLAG_WIN_SUM];
+    c->win_ok = c->win_n <= kWinMax;
+    return VOX_OK;
+}
+
+// The windows when the product variant in use (or being timed) reads them.
+static int prepare_product(vox_ctx* c, bool all_variants = false)
+{
+    if (all_variants || c->tune_spmv == 0) return ensure_window(c);
+    return VOX_OK;
+}
+
 template <class VT, class MT>
 static void fill_args(vox_ctx* c, CgArgs<VT, MT>& a, const MT* aval, VT* x, VT* r, VT* p, VT* q,
                       const double* rhs, double tol, int maxit)
 {
+    const bool win = c->win_ok && c->win_asm == c->asm_count;
+    a.wcol = win ? reinterpret_cast<const v4u*>(c->wcol) : nullptr;
+    a.wuniq = c->wuniq; a.wn = c->wn; a.win_n = c->win_n;
     a.nu = c->nu; a.cap = c->a_cap; a.tol = tol; a.maxit = maxit; a.it = 0;
     a.npq = grid_for(c->nu > 0 ? c->nu : 1);
     a.rowlen = c->rowlen; a.acol = c->acol; a.aval = aval; a.diag = c->adiag; a.rhs = rhs;
+    a.dcol = c->dcol_ok ? reinterpret_cast<const v4u*>(c->dcol) : nullptr; a.cbase = c->cbase;
     a.x = x; a.r = r; a.p = p; a.q = q;
     a.part = c->partials; a.scal = c->dscal; a.flags = c->dflags;
     a.z = nullptr; a.slen = c->lpc_slen; a.scol = c->lpc_scol; a.sval = c->lpc_sval; a.gd = c->lpc_gd;
@@ -661,6 +999,7 @@ static int finish(vox_ctx* c, vox_solve_info* info, int iters, int conv, double 
 
 static int solve_fp64(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 {
+    VOX_TRY(prepare_product(c));
     CgArgs<double, double> a;
     fill_args(c, a, (const double*)c->aval, c->cg_x, c->cg_r, c->cg_p, c->cg_q, c->rhs,
               c->p.cg_tol, maxit);
@@ -710,6 +1049,7 @@ static int solve_refined(vox_ctx* c, vox_solve_info* info, const MT* aval, VT* x
     const int nu = c->nu;
     const int npq = grid_for(nu > 0 ? nu : 1);
     if (npq > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
+    VOX_TRY(prepare_product(c));
     if (warm && nu > 0) {
         k_gather_x0<<<grid_for(nu), kBlock, 0, c->stream>>>(nu, c->urow, c->f[VOX_F_P], c->cg_x);
         VOX_HIP(hipGetLastError());
@@ -831,6 +1171,7 @@ int time_spmv_variants(vox_ctx* c, int reps, float* ms)
 {
     if (c->nu <= 0) return fail(VOX_ERR_STATE, "no assembled PPE to time the product on");
     if (grid_for(c->nu) > kMaxPartials) return fail(VOX_ERR_ARG, "PPE larger than the solver's row limit");
+    VOX_TRY(prepare_product(c, true));
     if (c->p.precision == VOX_PREC_MIXED) {
         if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
         CgArgs<float, float> a;

@@ -19,7 +19,11 @@ constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
 constexpr int kPartialSlots = 10;    // scalars reduced through ctx->partials
 constexpr int kPredVariants = 5;     // predictor: records per group x block size
-constexpr int kSpmvVariants = 3;     // PPE product: entries per load batch (8, 4, 16)
+constexpr int kSpmvVariants = 5;     // PPE product: 8 entries per batch gathered from an LDS
+                                     // window; 4, 16, 8 on 32-bit columns; 8 on 16-bit steps
+constexpr int kWinMax = 7936;        // distinct columns of a 256-row PPE chunk in the window
+constexpr int kWinSpanWords = 8192;  // bitmap words of the window build: column span <= 262,144
+                                     // (bitmap and word prefixes: 48 KiB of LDS at most)
 
 
 // Gather records: everything a neighbour loop reads about particle j in one
@@ -76,6 +80,11 @@ enum DevFlag {
     FLAG_CG_CONV,
     FLAG_CG_BREAK,
     FLAG_DOMAIN_FIRST,    // smallest index of a particle outside the grid (atomicMin)
+    FLAG_DCOL_BAD,        // a PPE row's column steps do not fit 16 bits (see dcol)
+    FLAG_WIN_N,           // most distinct columns of a PPE chunk (atomicMax; see wcol)
+    FLAG_WIN_SUM,         // distinct columns over all chunks (atomicAdd)
+    FLAG_COL_BELOW,       // max over PPE rows of row - first column (atomicMax)
+    FLAG_COL_ABOVE,       // max over PPE rows of last column - row (atomicMax)
     FLAG_COUNT
 };
 
@@ -139,8 +148,16 @@ __host__ __device__ __forceinline__ long mx_index(long r, int l, int cap)
 }
 
 typedef int32_t v4i __attribute__((ext_vector_type(4)));
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Chunk g (entries 8g..8g+7) of row r's 16-bit column steps, in 16-byte
+// units: a wave's 64 rows read chunk g as one contiguous 1 KiB block.
+__host__ __device__ inline long dc_index(long r, int g, int cap)
+{
+    return ((r >> 6) * (long)(cap >> 3) + g) * 64 + (r & 63);
+}
 
 // Non-temporal loads of 8 consecutive entries (p 32-byte aligned): streamed
 // data that should not displace gathered vectors from the caches.
@@ -240,6 +257,30 @@ struct vox_ctx {
     bool uorder_cells = false;     // unknowns numbered in cell order (else particle order)
     bool uorder_particles = false; // force particle order (a dump for the learned network)
     int32_t* acol = nullptr;       // cap_l * tp, mx_index
+    // The same columns as 16-bit steps: entry k of row r holds col_k - col_(k-1)
+    // (entry 0: 0, col_0 in cbase[r]); a lane's 8 steps of entries 8g..8g+7
+    // are one 16-byte chunk, chunk g of row r at ((r/64)(cap/8) + g) 64 + r%64.
+    // Valid when every step of the assembly fit (dcol_ok): the rows' columns
+    // ascend when the unknowns are numbered in the list's cell order.
+    uint32_t* dcol = nullptr;      // cap_l * tp 16-bit steps
+    int32_t* cbase = nullptr;      // tp: first column of each row
+    bool dcol_ok = false;
+    // Per chunk of 256 rows (one product block): the distinct columns its rows
+    // read, ascending (at most kWinMax, at wuniq + chunk kWinMax), and each
+    // entry's rank among them, 16-bit in dcol's layout (wcol). The product
+    // stages p at those columns in LDS once per block and gathers from there.
+    // Built on demand for the current assembly (win_asm == asm_count).
+    uint32_t* wcol = nullptr;      // cap_l * tp 16-bit ranks
+    int32_t* wuniq = nullptr;
+    int32_t* wn = nullptr;         // distinct columns per chunk
+    long wchunks_alloc = 0;
+    int wcap_alloc = 0;            // a_cap wcol was allocated for
+    int win_n = 0;                 // most distinct columns of a chunk
+    int win_sum = 0;               // distinct columns over all chunks
+    int col_below = 0, col_above = 0;   // every row's columns lie in [r - below, r + above]
+    bool win_ok = false;
+    long asm_count = 0;            // PPE assemblies and loads so far
+    long win_asm = -1;             // the asm_count the windows were built for
     double* aval = nullptr;        // cap_l * tp, mx_index
     double* adiag = nullptr;       // tp
     double* rhs = nullptr;         // tp

@@ -129,6 +129,9 @@ struct AsmArgs {
     const int32_t *bcon, *uidx, *urow;
     const int32_t* nu;                     // device: the number of unknowns
     int32_t *acol, *rowlen;
+    v4u* dcol;                             // 16-bit column steps (null: not stored)
+    int32_t *cbase, *dbad;                 //   first columns; set when a step does not fit
+    int32_t* creach;                       //   max row - first column, max last column - row
     double *aval, *adiag, *rhs, *source;
     float* aval32;                         // FP32 values (mixed precision, learned network)
     PosRec* rec;                           // the records the row entries index
@@ -188,6 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
     const long ob = mx_index(r, 0, a.acap);                  // entry k at ob + 64 k
     double diag = 0.0;
     int k = 0, l0 = 0;
+    int first = 0, last = 0;                                  // column steps
+    bool bad = false;
     const int cnt = live ? a.L.cnt[self] : 0;
     constexpr int U = 8;
     for (int g = 0;; ++g) {
@@ -234,12 +239,23 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
         if (!__any(has)) break;                                // every row is written
         if (has) {
             const int s0 = (g & 1) * 8;
+            // the group's columns also as steps from the previous column
+            // (entry 0 from itself: its column is the row's base); steps past
+            // the row's end are 0
+            int prev = (g == 0) ? rc[s0] : last;
+            if (g == 0) first = prev;
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int kk = 8 * g + u;
                 if (kk < k) {
                     const long o = ob + 64L * kk;
-                    a.acol[o] = rc[s0 + u];
+                    const int cu = rc[s0 + u];
+                    a.acol[o] = cu;
+                    const int d = cu - prev;
+                    bad = bad || d < 0 || d > 0xffff;
+                    w[u >> 1] |= (uint32_t)(d & 0xffff) << (16 * (u & 1));
+                    prev = cu;
                     if constexpr (sizeof(SV) == 8) {
                         a.aval[o] = rv[s0 + u];
                         if (a.aval32) a.aval32[o] = (float)rv[s0 + u];
@@ -248,12 +264,32 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
                     }
                 }
             }
+            last = prev;
+            if (a.dcol) a.dcol[dc_index(r, g, a.acap)] = v4u{w[0], w[1], w[2], w[3]};
+        }
+    }
+    if (a.dcol) {
+        // how far the rows' columns reach below and above the row (every lane
+        // of the wave is here: the loop above ends for all of them together)
+        int below = (live && k > 0) ? r - first : 0;
+        int above = (live && k > 0) ? last - r : 0;
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            below = max(below, __shfl_xor(below, o));
+            above = max(above, __shfl_xor(above, o));
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            atomicMax(&a.creach[0], below);
+            atomicMax(&a.creach[1], above);
         }
     }
     if (!live) return;
     a.adiag[r] = diag + a.dshift;
     a.rowlen[r] = k;
     a.rhs[r] = a.source[i];
+    if (a.dcol) {
+        a.cbase[r] = first;
+        if (bad) atomicOr(a.dbad, 1);
+    }
 }
 
 // FP64 copy of FP32-stored values (a solve in FP64 of a system assembled in
@@ -381,8 +417,10 @@ int ppe_assemble(vox_ctx* c)
     if (c->a_cap < L.cap) {
         if (c->acol) (void)hipFree(c->acol);
         if (c->aval) (void)hipFree(c->aval);
+        if (c->dcol) (void)hipFree(c->dcol);
         c->acol = nullptr;
         c->aval = nullptr;
+        c->dcol = nullptr;
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(n, L.cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(n, L.cap)));
         if (c->aval32) (void)hipFree(c->aval32);
@@ -419,6 +457,12 @@ int ppe_assemble(vox_ctx* c)
     a.z = c->f[star ? VOX_F_ZSTAR : VOX_F_Z]; a.pnd = c->f[star ? VOX_F_NSTAR : VOX_F_N];
     a.bcon = c->bcon; a.uidx = c->uidx; a.urow = c->urow; a.nu = c->dflags + FLAG_NU;
     a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
+    if (!c->dcol) VOX_HIP(hipMalloc(&c->dcol, sizeof(uint16_t) * (size_t)sl_size(n, c->a_cap)));
+    if (!c->cbase) VOX_HIP(hipMalloc(&c->cbase, sizeof(int32_t) * (size_t)n));
+    a.dcol = reinterpret_cast<v4u*>(c->dcol); a.cbase = c->cbase; a.dbad = c->dflags + FLAG_DCOL_BAD;
+    a.creach = c->dflags + FLAG_COL_BELOW;
+    VOX_TRY(fill_i32(c, a.dbad, 0, 1));
+    VOX_TRY(fill_i32(c, a.creach, 0, 2));
     a.aval32 = mixed ? c->aval32 : nullptr;
     a.adiag = c->adiag; a.rhs = c->rhs;
     a.source = c->f[VOX_F_SOURCE];
@@ -442,6 +486,10 @@ int ppe_assemble(vox_ctx* c)
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
+    c->dcol_ok = !c->hflags[FLAG_DCOL_BAD];
+    c->col_below = c->hflags[FLAG_COL_BELOW];
+    c->col_above = c->hflags[FLAG_COL_ABOVE];
+    c->asm_count += 1;
     c->aval32_ok = mixed;
     c->aval64_ok = !fp32_only;
     if (p.srcopt) {
@@ -619,7 +667,9 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
         VOX_HIP(hipMalloc(&c->acol, sizeof(int32_t) * (size_t)sl_size(c->tp, cap)));
         VOX_HIP(hipMalloc(&c->aval, sizeof(double) * (size_t)sl_size(c->tp, cap)));
         if (c->aval32) (void)hipFree(c->aval32);
+        if (c->dcol) (void)hipFree(c->dcol);
         c->aval32 = nullptr;
+        c->dcol = nullptr;
         c->a_cap = cap;
     }
     const long tot = sl_size(nu, c->a_cap);
@@ -649,6 +699,8 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
     c->nu = nu;
     c->aval32_ok = false;
     c->aval64_ok = true;
+    c->dcol_ok = false;            // the product reads the 32-bit columns
+    c->asm_count += 1;
     c->uorder_cells = false;       // the rows keep the loaded order
     return VOX_OK;
 }
