@@ -284,7 +284,9 @@ int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const doub
  * cell-sorted particles) over `reps` builds on the current positions and keeps the faster per
  * list (modes[2] receives the choice for lists S and L); then runs the step's
  * stages up to MATRIX once and times the predictor's variants (FP64: records
- * gathered per group 4, 8 or 2 with 256 threads per block, 4 or 8 with 128)
+ * gathered per group 4, 8 or 2 with 256 threads per block, 4 or 8 with 128;
+ * variants 5-8: 8 or 4 records with 128 or 256 threads and the gradient terms
+ * deferred to a queue)
  * and the PPE product's (0: 8 entries per load batch with p gathered from a
  * per-block LDS window of the block's distinct columns; 1, 2, 3: 4, 16 and 8
  * entries on the 32-bit columns; 4: 8 entries on 16-bit column steps; 0 and 4
@@ -293,7 +295,7 @@ int vox_set_regions(vox_ctx* ctx, int nregion, const int32_t* region, const doub
  * vox_init. */
 int vox_autotune(vox_ctx* ctx, int reps, int32_t* modes);
 /* The context's tuning (no counterpart in the reference): choice[4] = walk of
- * list S, walk of list L (0 = heuristic), predictor variant (0..4), product
+ * list S, walk of list L (0 = heuristic), predictor variant (0..8), product
  * variant (0..4); vox_set_tuning applies a cached choice without timing. */
 int vox_get_tuning(vox_ctx* ctx, int32_t* choice);
 int vox_set_tuning(vox_ctx* ctx, const int32_t* choice);

@@ -1,5 +1,6 @@
 """The autotuner's kernel variants give the same bits: every predictor
-variant (records gathered per group x threads per block) and every PPE
+variant (records gathered per group x threads per block, gradient terms in
+the row walk or deferred to a queue) and every PPE
 product variant (entries per load batch; columns as 16-bit steps or 32-bit
 ids) against the default on the same state, and a tuned run against an
 untuned one."""
@@ -26,7 +27,7 @@ def test_kernel_variants_are_bit_identical(hip, precision):
     for st in ("PNUM", "NEIGHBOR", "V_FRACTION", "VISCOSITY"):
         getattr(s, st)()
     ref = None
-    for v in range(5):
+    for v in range(9):
         s.set_tuning([0, 0, v, 0])
         s.PREDICTION()
         out = [s.get(n) for n in ("XSTAR", "YSTAR", "ZSTAR", "USTAR", "VSTAR", "WSTAR")]
@@ -53,7 +54,7 @@ def test_kernel_variants_are_bit_identical(hip, precision):
             ref = out
         assert np.array_equal(out[0], ref[0]) and out[1] == ref[1], f"product variant {v}"
     with pytest.raises(vx.VoxError):
-        s.set_tuning([0, 0, 5, 0])
+        s.set_tuning([0, 0, 9, 0])
     with pytest.raises(vx.VoxError):
         s.set_tuning([0, 0, 0, 5])
     s.close()
@@ -69,7 +70,7 @@ def test_autotuned_run_matches_untuned(hip):
     b.autotune(reps=2)
     t = b.tuning()
     print("tuning (walk S, walk L, predictor, product):", t)
-    assert t[0] in (1, 2) and t[1] in (1, 2) and 0 <= t[2] < 5 and 0 <= t[3] < 5
+    assert t[0] in (1, 2) and t[1] in (1, 2) and 0 <= t[2] < 9 and 0 <= t[3] < 5
     a.step(10)
     b.step(10)
     for n in ("X", "Y", "Z", "U", "V", "W", "P"):

@@ -18,7 +18,7 @@ constexpr int kBlock = 256;          // threads per workgroup (4 waves)
 constexpr int kWave = 64;            // CDNA wavefront width
 constexpr int kMaxPartials = 8192;   // reduction partial slots per scalar
 constexpr int kPartialSlots = 10;    // scalars reduced through ctx->partials
-constexpr int kPredVariants = 5;     // predictor: records per group x block size
+constexpr int kPredVariants = 9;     // predictor: records per group x block size x deferred gradient
 constexpr int kSpmvVariants = 5;     // PPE product: 8 entries per batch gathered from an LDS
                                      // window; 4, 16, 8 on 32-bit columns; 8 on 16-bit steps
 constexpr int kWinMax = 7936;        // distinct columns of a 256-row PPE chunk in the window
@@ -232,7 +232,8 @@ struct vox_ctx {
 
     vox::DevList list[2];
     int tuned_mode[2] = {0, 0};    // neighbour walk chosen by the autotuner (0: none)
-    int tune_pred = 0;             // predictor variant (vox::kPredVariants)
+    int tune_pred = 6;             // predictor variant (vox::kPredVariants): 8 records,
+                                   // 256 threads, gradient deferred unless tuned
     int tune_spmv = 0;             // PPE product variant (vox::kSpmvVariants)
     bool two_lists = false;
     double n0s = 0.0, n0l = 0.0;

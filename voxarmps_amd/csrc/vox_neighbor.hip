@@ -464,16 +464,21 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
     const double R2 = R * R;
     const double lo = R2 * (1.0 - 1e-10), hi = R2 * (1.0 + 1e-10);
     int k = 0;
+    const int tt = (int)t;
     unsigned long long todo = __ballot(valid);
     while (todo) {
+        // the group's cell row and x-range are the same for every lane: kept
+        // in scalar registers, so the walk's cell bounds and the candidate
+        // index are scalar too
         const int lead = __ffsll(todo) - 1;
-        const int grow = __shfl(rowkey, lead);
-        const int gx = __shfl(ix, lead);
+        const int grow = __builtin_amdgcn_readfirstlane(__shfl(rowkey, lead));
+        const int gx = __builtin_amdgcn_readfirstlane(__shfl(ix, lead));
         const bool mine = valid && rowkey == grow && ix <= gx + 1 && ((todo >> lane) & 1ull);
         const unsigned long long m = __ballot(mine);
         todo &= ~m;
         const int last = 63 - __clzll(m);
-        const int x_lo = max(gx - 1, 0), x_hi = min(__shfl(ix, last) + 1, g.ncx - 1);
+        const int x_lo = max(gx - 1, 0);
+        const int x_hi = min(__builtin_amdgcn_readfirstlane(__shfl(ix, last)) + 1, g.ncx - 1);
         const int gy = grow % g.ncy, gz = grow / g.ncy;
         const int y_lo = max(gy - 1, 0), y_hi = min(gy + 1, g.ncy - 1);
         const int z_lo = (g.dim == 3) ? max(gz - 1, 0) : 0;
@@ -483,7 +488,8 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
             for (int yb = y_lo; yb <= y_hi; ++yb)
                 for (int zb = z_lo; zb <= z_hi; ++zb) {
                     const long cn = xb + yb * (long)g.ncx + zb * ncxy;
-                    const int cs = start[cn], ce = start[cn + 1];
+                    const int cs = __builtin_amdgcn_readfirstlane(start[cn]);
+                    const int ce = __builtin_amdgcn_readfirstlane(start[cn + 1]);
                     for (int j0 = cs; j0 < ce; j0 += kWave) {
                         const int nc = min(kWave, ce - j0);
                         __syncthreads();                   // last chunk's reads are done
@@ -510,7 +516,7 @@ __global__ __launch_bounds__(kWave) void k_build_list_xrow(
                                 }
                                 // the particle itself: by its position in rec (rows
                                 // of positions), else by id
-                                const bool other = kPos ? (j0 + u != t) : (tile[u].id != i);
+                                const bool other = kPos ? (j0 + u != tt) : (tile[u].id != i);
                                 acc = acc && on && (u < nc) && (g.incl || other);
                                 // the next free slot takes every candidate; only
                                 // an accepted one advances past it

@@ -561,6 +561,146 @@ __global__ __launch_bounds__(BS) void k_prediction(PredArgs a)
     }
 }
 
+// The same predictor with the gradient terms deferred: the row walk adds the
+// Laplacian terms of every entry and queues (in LDS, in row order) the
+// entries within re_s, whose gradient terms are then added in a second pass
+// over the queue, the distance recomputed by the same expression. Each of the
+// two groups of sums keeps its terms in row order, so the results are the
+// same bits. In the single walk a wave runs the gradient block whenever any
+// of its 64 lanes has an entry within re_s at that row index (most of them),
+// here once per queued entry of its busiest lane (about 1 in 7 entries).
+constexpr int kGradQ = 48;   // queued entries per lane before the queue is drained
+
+template <int BS>
+__device__ __forceinline__ void pred_grad_drain(const PredArgs& a, const int32_t* gq, int nq,
+                                                double xi, double yi, double zi, double phi_,
+                                                double inv_re_s, double& s1, double& s2,
+                                                double& s3)
+{
+    constexpr int G = 8;
+    for (int q0 = 0; __any(q0 < nq); q0 += G) {
+        int jj[G];
+        PredRec rv[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) jj[u] = (q0 + u < nq) ? gq[(q0 + u) * BS] : -1;
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+            if (jj[u] >= 0) rv[u] = a.rec[jj[u]];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            if (jj[u] >= 0) {
+                const PredRec rj = rv[u];
+                const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
+                const double D = sqrt(dx * dx + dy * dy + dz * dz);
+                const double dp = rj.pn - phi_;
+                const double inv_D = 1.0 / D;
+                const double wg = wfun_r(D, a.ktype, a.re_s, inv_re_s);
+                s1 = s1 + div_by(div_by(dp * dx * wg, D, inv_D), D, inv_D);
+                s2 = s2 + div_by(div_by(dp * dy * wg, D, inv_D), D, inv_D);
+                s3 = s3 + div_by(div_by(dp * dz * wg, D, inv_D), D, inv_D);
+            }
+        }
+    }
+}
+
+// (specialised on the kernel type, 8 records in flight fit in 3 waves per
+// SIMD, 166 VGPRs against 188, and measured no faster)
+template <bool kUniform, int R, int BS>
+__global__ __launch_bounds__(BS) void k_prediction_dg(PredArgs a)
+{
+    __shared__ int32_t gq_all[kGradQ * BS];                 // slot-major: conflict-free
+    int32_t* gq = gq_all + threadIdx.x;                     // slot q at gq[q * BS]
+    const int t = xcd_tid();
+    if (t >= a.n) return;
+    const int i = sorted_particle(a.order, t);
+    const int self = a.pos ? t : i;
+    const int32_t* __restrict__ rpt = a.pos ? a.rpt : a.pt;
+    const int dim = a.dim;
+    const int ti = a.pt[i];
+    if (ti <= 0) {
+        a.xs[i] = a.x[i]; a.ys[i] = a.y[i]; a.us[i] = a.u[i]; a.vs[i] = a.v[i];
+        if (dim == 3) { a.zs[i] = a.z[i]; a.ws[i] = a.w[i]; }
+        return;
+    }
+    const PredRec me = a.rec[self];
+    const double xi = me.x, yi = me.y, zi = me.z, ui = me.u, vi = me.v, wi = me.w;
+    const double mi = me.meu, phi_ = a.ph[i];
+    double rhoN = (ti == 1) ? a.rho1 : a.rho2;
+    const double inv_rhoN = 1.0 / rhoN;
+    const double inv_re_s = 1.0 / a.re_s, inv_re_l = 1.0 / a.re_l;
+    double s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0;
+    int nq = 0;
+    const int cnt = a.L.cnt[self];
+    const int32_t* __restrict__ row = a.L.ids + sl_index(self, 0, a.L.cap);
+    // one drain site (the loop runs to the wave's longest row; a lane past
+    // its own row only drains)
+    for (int g = 0;; ++g) {
+    const bool more = (g << 3) < cnt;
+    if (more) {
+    int32_t jc[8];
+    load_chunk8(row, g, jc);
+#pragma unroll
+    for (int h = 0; h < 8 / R; ++h) {
+    constexpr int U = R;
+    int jj[U], tj[U];
+    PredRec rv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jj[u] = ((g << 3) + U * h + u < cnt) ? jc[U * h + u] : self;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        rv[u] = a.rec[jj[u]];
+        tj[u] = kUniform ? ti : rpt[jj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const PredRec rj = rv[u];
+        const double dx = rj.x - xi, dy = rj.y - yi, dz = rj.z - zi;
+        const double D = sqrt(dx * dx + dy * dy + dz * dz);
+        const bool use = (jj[u] != self) && (D > 0);
+        const double mj = rj.meu;
+        const double NEU = (kUniform || ti == tj[u])
+                               ? div_by(mi, rhoN, inv_rhoN)
+                               : div_by(2 * mi * mj / (mi + mj), rhoN, inv_rhoN);
+        const double du = rj.u - ui, dv = rj.v - vi, dw = rj.w - wi;
+        // W(D; re_s) is 0 beyond re_s (every kernel type), where the
+        // gradient terms are +-0 and leave the sums' bits unchanged
+        if (use && div_by(D, a.re_s, inv_re_s) <= 1.0) {
+            gq[nq * BS] = jj[u];
+            ++nq;
+        }
+        const double wl = wfun_r(D, a.ktype, a.re_l, inv_re_l);
+        s4 = use ? s4 + wl * du * NEU : s4;
+        s5 = use ? s5 + wl * dv * NEU : s5;
+        s7 = use ? s7 + wl * du : s7;
+        s8 = use ? s8 + wl * dv : s8;
+        s6 = use ? s6 + wl * dw * NEU : s6;
+        s9 = use ? s9 + wl * dw : s9;
+    }
+    }
+    }
+    const bool last = !__any(more);
+    if (last || __any(nq > kGradQ - 8)) {                   // room for the next chunk
+        pred_grad_drain<BS>(a, gq, nq, xi, yi, zi, phi_, inv_re_s, s1, s2, s3);
+        nq = 0;
+    }
+    if (last) break;
+    }
+    double nt = a.neut[i];
+    double gc = a.relax1 * (a.gcoef_num * a.dt / a.n0s / a.rho[i]);
+    double dt = a.dt;
+    double us = ui + a.gx * dt + a.lapc * dt * (s4 + nt * s7) / a.ln0 + dt * a.t1[i] - gc * s1;
+    double vs = vi + a.gy * dt + a.lapc * dt * (s5 + nt * s8) / a.ln0 + dt * a.t2[i] - gc * s2;
+    a.us[i] = us;
+    a.vs[i] = vs;
+    a.xs[i] = xi + dt * us;
+    a.ys[i] = yi + dt * vs;
+    if (dim == 3) {
+        double ws = wi + a.gz * dt + a.lapc * dt * (s6 + nt * s9) / a.ln0 + dt * a.t3[i] - gc * s3;
+        a.ws[i] = ws;
+        a.zs[i] = zi + dt * ws;
+    }
+}
+
 // Every MEU equals nu1 rho1 (Newtonian viscosity and no PTYPE 2 particle:
 // functions.cu:2073-2085) and the harmonic mean 2 m m / (m + m) of that value
 // rounds back to it, so NEU does not depend on the neighbour's type.
@@ -572,14 +712,19 @@ bool uniform_meu(const vox_ctx* c)
     return hm == m;
 }
 
-// Predictor variants for the autotuner: (records per group, block size).
-static const int kPredR[kPredVariants] = {4, 8, 2, 4, 8};
-static const int kPredBS[kPredVariants] = {256, 256, 256, 128, 128};
+// Predictor variants for the autotuner: (records per group, block size), the
+// gradient terms in the row walk (0-4) or deferred to a queue (5-8).
+static const int kPredR[kPredVariants] = {4, 8, 2, 4, 8, 8, 8, 4, 4};
+static const int kPredBS[kPredVariants] = {256, 256, 256, 128, 128, 128, 256, 128, 256};
 
 template <bool kU>
 static void launch_pred_u(const PredArgs& a, int v, int n, hipStream_t s)
 {
     switch (v) {
+    case 5: k_prediction_dg<kU, 8, 128><<<grid_for(n, 128), 128, 0, s>>>(a); break;
+    case 6: k_prediction_dg<kU, 8, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
+    case 7: k_prediction_dg<kU, 4, 128><<<grid_for(n, 128), 128, 0, s>>>(a); break;
+    case 8: k_prediction_dg<kU, 4, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
     case 1: k_prediction<kU, 8, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
     case 2: k_prediction<kU, 2, 256><<<grid_for(n, 256), 256, 0, s>>>(a); break;
     case 3: k_prediction<kU, 4, 128><<<grid_for(n, 128), 128, 0, s>>>(a); break;
@@ -602,8 +747,8 @@ const char* pred_variant_name(int v)
 {
     static char buf[kPredVariants][48];
     if (v < 0 || v >= kPredVariants) return "?";
-    std::snprintf(buf[v], sizeof buf[v], "%d records per group, %d threads per block", kPredR[v],
-                  kPredBS[v]);
+    std::snprintf(buf[v], sizeof buf[v], "%d records per group, %d threads per block%s", kPredR[v],
+                  kPredBS[v], v >= 5 ? ", gradient deferred" : "");
     return buf[v];
 }
 
