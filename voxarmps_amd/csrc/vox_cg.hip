@@ -674,9 +674,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_p(CgArgs<VT, MT> a)
 // ---- outer FP64 refinement kernels ---------------------------------------------------------
 template <class MT>
 struct IrArgs {
-    int nu, cap, npq;
+    int nu, cap, npq, win_n;
     const int32_t* __restrict__ rowlen;
     const int32_t* __restrict__ acol;
+    const v4u* __restrict__ wcol;     // ranks in the product's LDS windows (see k_cg_spmv_win)
+    const int32_t* __restrict__ wuniq;
+    const int32_t* __restrict__ wn;
     const MT* __restrict__ aval;
     const double* __restrict__ diag;
     const double* __restrict__ b;
@@ -716,6 +719,54 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs<MT> a)
             for (int u = 0; u < U; ++u) acc = acc + vv[u] * xx[u];
         }
         for (; k < len; ++k) acc = acc + (double)val[64L * k] * a.x[col[64L * k]];
+        const double b = a.b[r];
+        const double rv = b - acc;
+        a.res[r] = rv;
+        rr = rv * rv;
+        bb = b * b;
+    }
+    rr = block_sum(rr, lds);
+    bb = block_sum(bb, lds);
+    if (threadIdx.x == 0) {
+        PSLOT(a, P_RES)[blockIdx.x] = rr;
+        PSLOT(a, P_RHS)[blockIdx.x] = bb;
+    }
+}
+
+// The same residual with x gathered from the product's LDS windows.
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_ir_residual_win(IrArgs<MT> a)
+{
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    double* win = reinterpret_cast<double*>(dyn_lds);
+    __shared__ double lds[kBlock];
+    const int ch = xcd_block();
+    const int n = a.wn[ch];
+    const int32_t* __restrict__ uq = a.wuniq + (long)ch * kWinMax;
+    for (int t = threadIdx.x; t < n; t += kBlock) win[t] = a.x[uq[t]];
+    __syncthreads();
+    double rr = 0.0, bb = 0.0;
+    const int r = ch * kBlock + threadIdx.x;
+    if (r < a.nu) {
+        double acc = a.diag[r] * a.x[r];
+        const int len = a.rowlen[r];
+        const v4u* __restrict__ wc = a.wcol + dc_index(r, 0, a.cap);
+        const MT* __restrict__ val = a.aval + mx_index(r, 0, a.cap);
+        int k = 0;
+        for (; k + 8 <= len; k += 8) {
+            const v4u w = __builtin_nontemporal_load(&wc[8L * k]);
+            double vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) vv[u] = (double)__builtin_nontemporal_load(&val[64L * (k + u)]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + vv[u] * win[u16_of(w, u)];
+        }
+        if (k < len) {
+            const v4u w = __builtin_nontemporal_load(&wc[8L * k]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < len) acc = acc + (double)val[64L * (k + u)] * win[u16_of(w, u)];
+        }
         const double b = a.b[r];
         const double rv = b - acc;
         a.res[r] = rv;
@@ -1014,23 +1065,32 @@ static int solve_fp64(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 
 // res = b - A x (FP64) over the matrix as stored, then the norms into the
 // scalar slots; FP64 values when the assembly kept them, else its FP32 values.
+template <class MT>
+static void launch_ir(vox_ctx* c, IrArgs<MT>& ir, const MT* aval, int npq)
+{
+    ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
+    ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = aval; ir.diag = c->adiag;
+    ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
+    const bool win = c->win_ok && c->win_asm == c->asm_count;
+    ir.wcol = win ? reinterpret_cast<const v4u*>(c->wcol) : nullptr;
+    ir.wuniq = c->wuniq; ir.wn = c->wn; ir.win_n = c->win_n;
+    if (win)
+        k_ir_residual_win<MT><<<npq, kBlock, sizeof(double) * (size_t)ir.win_n, c->stream>>>(ir);
+    else
+        k_ir_residual<MT><<<npq, kBlock, 0, c->stream>>>(ir);
+    k_ir_residual_fin<MT><<<1, kBlock, 0, c->stream>>>(ir);
+}
+
+// (x gathered from the product's LDS windows when they are built)
 static int ir_residual(vox_ctx* c, int npq)
 {
     ProfScope ps(c, KC_IR);
     if (c->aval64_ok) {
         IrArgs<double> ir;
-        ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
-        ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval; ir.diag = c->adiag;
-        ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
-        k_ir_residual<double><<<npq, kBlock, 0, c->stream>>>(ir);
-        k_ir_residual_fin<double><<<1, kBlock, 0, c->stream>>>(ir);
+        launch_ir(c, ir, (const double*)c->aval, npq);
     } else {
         IrArgs<float> ir;
-        ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
-        ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = c->aval32; ir.diag = c->adiag;
-        ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
-        k_ir_residual<float><<<npq, kBlock, 0, c->stream>>>(ir);
-        k_ir_residual_fin<float><<<1, kBlock, 0, c->stream>>>(ir);
+        launch_ir(c, ir, (const float*)c->aval32, npq);
     }
     VOX_HIP(hipGetLastError());
     return VOX_OK;
