@@ -283,7 +283,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_win(CgArgs<VT, MT> a)
 struct WinArgs {
     int nu, cap, words;        // words: bitmap words in the dynamic LDS
     const int32_t* __restrict__ rowlen;
-    const int32_t* __restrict__ acol;
     const v4u* __restrict__ dcol;
     const int32_t* __restrict__ cbase;
     v4u* wcol;
@@ -293,13 +292,13 @@ struct WinArgs {
 };
 
 // The windows of the LDS-gather product, one block per chunk of 256 rows
-// (rows ascending, so the chunk's columns span [smallest first column,
-// largest last column]): a bitmap of the columns the rows use (in dynamic
-// LDS, `words` words, sized from the assembly's bound on the rows' reach),
-// the prefix of its popcounts per word, the distinct columns in ascending
-// order and each entry's rank among them. A chunk whose span exceeds the
-// bitmap or whose distinct columns exceed kWinMax raises FLAG_WIN_N past
-// kWinMax, and the product then reads the columns themselves. Each lane walks
+// (rows ascending): a bitmap of the columns the rows use, based at the
+// chunk's smallest first column (in dynamic LDS, `words` words, sized from
+// the assembly's bound on how far the rows reach below and above their own
+// index), the prefix of its popcounts per word, the distinct columns in
+// ascending order and each entry's rank among them. A chunk with a column
+// past the bitmap or more distinct columns than kWinMax raises FLAG_WIN_N
+// past kWinMax, and the product then reads the column steps. Each lane walks
 // its row 4 chunks (32 entries) per round trip.
 __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
 {
@@ -307,12 +306,13 @@ __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
     uint32_t* bits = wlds;
     uint16_t* pre = reinterpret_cast<uint16_t*>(wlds + a.words);
     __shared__ int red[kBlock];
+    __shared__ int outside;
     const int ch = blockIdx.x, t = threadIdx.x;
     const int r = ch * kBlock + t;
     const int len = (r < a.nu) ? a.rowlen[r] : 0;
     const int first = (len > 0) ? a.cbase[r] : 0x7fffffff;
-    const int lastc = (len > 0) ? a.acol[mx_index(r, len - 1, a.cap)] : -1;
     red[t] = first;
+    if (t == 0) outside = 0;
     __syncthreads();
     for (int s = kBlock / 2; s > 0; s >>= 1) {
         if (t < s) red[t] = min(red[t], red[t + s]);
@@ -320,24 +320,12 @@ __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
     }
     const int lo = red[0];
     __syncthreads();
-    red[t] = lastc;
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (t < s) red[t] = max(red[t], red[t + s]);
-        __syncthreads();
-    }
-    const int hi = red[0];
-    __syncthreads();
-    if (hi < lo) {                                     // no entries (uniform)
+    if (lo == 0x7fffffff) {                            // no entries (uniform)
         if (t == 0) a.wn[ch] = 0;
         return;
     }
-    const long span = (long)hi - lo + 1;
-    if (span > 32L * a.words) {                        // uniform
-        if (t == 0) { a.wn[ch] = -1; atomicMax(&a.flags[FLAG_WIN_N], kWinMax + 1); }
-        return;
-    }
-    const int nw = (int)((span + 31) >> 5);
+    const int nw = a.words;
+    const long nbits = 32L * nw;
     for (int w = t; w < nw; w += kBlock) bits[w] = 0u;
     __syncthreads();
     const v4u* __restrict__ dc = a.dcol + dc_index(r, 0, a.cap);
@@ -357,10 +345,14 @@ __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
             for (int u = 0; u < 8; ++u) {
                 if (k + 8 * q + u < len) {
                     cj += u16_of(w[q], u);
-                    const int o = cj - lo;
-                    if ((o >> 5) != cw) {
+                    const long o = (long)cj - lo;
+                    if (o >= nbits) {
+                        outside = 1;
+                        continue;
+                    }
+                    if ((int)(o >> 5) != cw) {
                         if (cw >= 0) atomicOr(&bits[cw], cm);
-                        cw = o >> 5;
+                        cw = (int)(o >> 5);
                         cm = 0u;
                     }
                     cm |= 1u << (o & 31);
@@ -370,6 +362,10 @@ __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
     }
     if (cw >= 0) atomicOr(&bits[cw], cm);
     __syncthreads();
+    if (outside) {                                     // uniform
+        if (t == 0) { a.wn[ch] = -1; atomicMax(&a.flags[FLAG_WIN_N], kWinMax + 1); }
+        return;
+    }
     // exclusive prefix of the words' popcounts: thread t owns words [w0, w1)
     const int per = (nw + kBlock - 1) / kBlock;
     const int w0 = min(nw, t * per), w1 = min(nw, w0 + per);
@@ -982,7 +978,7 @@ static int ensure_window(vox_ctx* c)
     if (words > kWinSpanWords) return VOX_OK;
     VOX_TRY(fill_i32(c, c->dflags + FLAG_WIN_N, 0, 2));          // and FLAG_WIN_SUM
     WinArgs w;
-    w.nu = c->nu; w.cap = c->a_cap; w.words = words; w.rowlen = c->rowlen; w.acol = c->acol;
+    w.nu = c->nu; w.cap = c->a_cap; w.words = words; w.rowlen = c->rowlen;
     w.dcol = reinterpret_cast<const v4u*>(c->dcol); w.cbase = c->cbase;
     w.wcol = reinterpret_cast<v4u*>(c->wcol); w.wuniq = c->wuniq; w.wn = c->wn; w.flags = c->dflags;
     {
@@ -998,10 +994,13 @@ static int ensure_window(vox_ctx* c)
     return VOX_OK;
 }
 
-// The windows when the product variant in use (or being timed) reads them.
+// The windows when the product variant in use (or being timed) reads them,
+// and the 32-bit columns when it reads those.
 static int prepare_product(vox_ctx* c, bool all_variants = false)
 {
-    if (all_variants || c->tune_spmv == 0) return ensure_window(c);
+    const int v = c->tune_spmv;
+    if (all_variants || v == 1 || v == 2 || v == 3 || !c->dcol_ok) VOX_TRY(ensure_acol(c));
+    if (all_variants || v == 0) return ensure_window(c);
     return VOX_OK;
 }
 
@@ -1066,12 +1065,13 @@ static int solve_fp64(vox_ctx* c, vox_solve_info* info, bool fcg, int maxit)
 // res = b - A x (FP64) over the matrix as stored, then the norms into the
 // scalar slots; FP64 values when the assembly kept them, else its FP32 values.
 template <class MT>
-static void launch_ir(vox_ctx* c, IrArgs<MT>& ir, const MT* aval, int npq)
+static int launch_ir(vox_ctx* c, IrArgs<MT>& ir, const MT* aval, int npq)
 {
     ir.nu = c->nu; ir.cap = c->a_cap; ir.npq = npq;
     ir.rowlen = c->rowlen; ir.acol = c->acol; ir.aval = aval; ir.diag = c->adiag;
     ir.b = c->rhs; ir.x = c->cg_x; ir.res = c->cg_r; ir.part = c->partials; ir.scal = c->dscal;
     const bool win = c->win_ok && c->win_asm == c->asm_count;
+    if (!win) VOX_TRY(ensure_acol(c));
     ir.wcol = win ? reinterpret_cast<const v4u*>(c->wcol) : nullptr;
     ir.wuniq = c->wuniq; ir.wn = c->wn; ir.win_n = c->win_n;
     if (win)
@@ -1079,6 +1079,7 @@ static void launch_ir(vox_ctx* c, IrArgs<MT>& ir, const MT* aval, int npq)
     else
         k_ir_residual<MT><<<npq, kBlock, 0, c->stream>>>(ir);
     k_ir_residual_fin<MT><<<1, kBlock, 0, c->stream>>>(ir);
+    return VOX_OK;
 }
 
 // (x gathered from the product's LDS windows when they are built)
@@ -1087,10 +1088,10 @@ static int ir_residual(vox_ctx* c, int npq)
     ProfScope ps(c, KC_IR);
     if (c->aval64_ok) {
         IrArgs<double> ir;
-        launch_ir(c, ir, (const double*)c->aval, npq);
+        VOX_TRY(launch_ir(c, ir, (const double*)c->aval, npq));
     } else {
         IrArgs<float> ir;
-        launch_ir(c, ir, (const float*)c->aval32, npq);
+        VOX_TRY(launch_ir(c, ir, (const float*)c->aval32, npq));
     }
     VOX_HIP(hipGetLastError());
     return VOX_OK;

@@ -266,6 +266,9 @@ struct vox_ctx {
     uint32_t* dcol = nullptr;      // cap_l * tp 16-bit steps
     int32_t* cbase = nullptr;      // tp: first column of each row
     bool dcol_ok = false;
+    bool acol_ok = false;          // acol holds the current system's columns (the
+                                   // assembly stores them only where the steps cannot
+                                   // stand in; ensure_acol decodes them on demand)
     // Per chunk of 256 rows (one product block): the distinct columns its rows
     // read, ascending (at most kWinMax, at wuniq + chunk kWinMax), and each
     // entry's rank among them, 16-bit in dcol's layout (wcol). The product
@@ -424,6 +427,7 @@ inline const DevList& list_of(const vox_ctx* c, int which)
 
 // host-side helpers implemented in the .hip files
 int scan_exclusive(vox_ctx* c, const int32_t* in, int32_t* out, long n, int32_t* total_dev);
+int ensure_acol(vox_ctx* c);                                    // vox_ppe.hip
 int fill_i32(vox_ctx* c, int32_t* p, int32_t v, long n);
 int fill_f64(vox_ctx* c, double* p, double v, long n);
 int read_flags(vox_ctx* c);   // D2H of dflags + dscal into pinned mirrors (syncs)

@@ -971,6 +971,7 @@ static void ctx_features(vox_ctx* c, int rows, v4f* upos, float* invd, float* fe
 
 int lpc_prepare(vox_ctx* c)
 {
+    VOX_TRY(ensure_acol(c));
     if (!c->lpc) return fail(VOX_ERR_STATE, "learned solver without an attached vox_lpc");
     if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
     ProfScope ps(c, KC_LPC_NET);
@@ -1116,6 +1117,7 @@ int vox_lpc_dump(vox_lpc* L, vox_ctx* c, int32_t* index)
         VOX_TRY(rc0);
     }
     if (!c->aval32_ok) VOX_TRY(ppe_make_aval32(c));
+    VOX_TRY(ensure_acol(c));
     LpcSys S;
     S.nu = c->nu; S.rows = round64(c->nu); S.cap = c->a_cap;
     const size_t nn = (size_t)S.rows, mat = (size_t)sl_size(S.nu, S.cap);

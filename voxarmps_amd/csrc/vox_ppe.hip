@@ -128,7 +128,7 @@ struct AsmArgs {
     const double *x, *y, *z, *pnd;
     const int32_t *bcon, *uidx, *urow;
     const int32_t* nu;                     // device: the number of unknowns
-    int32_t *acol, *rowlen;
+    int32_t *acol, *rowlen;                // acol null: the columns are not stored
     v4u* dcol;                             // 16-bit column steps (null: not stored)
     int32_t *cbase, *dbad;                 //   first columns; set when a step does not fit
     int32_t* creach;                       //   max row - first column, max last column - row
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
                 if (kk < k) {
                     const long o = ob + 64L * kk;
                     const int cu = rc[s0 + u];
-                    a.acol[o] = cu;
+                    if (a.acol) a.acol[o] = cu;
                     const int d = cu - prev;
                     bad = bad || d < 0 || d > 0xffff;
                     w[u >> 1] |= (uint32_t)(d & 0xffff) << (16 * (u & 1));
@@ -408,6 +408,19 @@ __global__ void k_rhs_from_source(int nu, const int32_t* __restrict__ urow, cons
     if (r < nu) rhs[r] = source[urow[r]];
 }
 
+static void launch_assemble_rows(vox_ctx* c, const AsmArgs& a, bool fp32_only)
+{
+    const int n = c->tp;
+    switch (kt_of(c->p.ktype) * 2 + (fp32_only ? 1 : 0)) {
+    case 4: k_assemble_rows<double, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 5: k_assemble_rows<float, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 12: k_assemble_rows<double, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 13: k_assemble_rows<float, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    case 1: k_assemble_rows<float, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    default: k_assemble_rows<double, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
+    }
+}
+
 int ppe_assemble(vox_ctx* c)
 {
     VOX_TRY(list_l_for_consumer(c));
@@ -456,7 +469,11 @@ int ppe_assemble(vox_ctx* c)
     a.x = c->f[star ? VOX_F_XSTAR : VOX_F_X]; a.y = c->f[star ? VOX_F_YSTAR : VOX_F_Y];
     a.z = c->f[star ? VOX_F_ZSTAR : VOX_F_Z]; a.pnd = c->f[star ? VOX_F_NSTAR : VOX_F_N];
     a.bcon = c->bcon; a.uidx = c->uidx; a.urow = c->urow; a.nu = c->dflags + FLAG_NU;
-    a.acol = c->acol; a.rowlen = c->rowlen; a.aval = c->aval;
+    // The 32-bit columns are stored where the 16-bit steps may not fit (the
+    // unknowns in particle order) or the learned preconditioner reads them;
+    // otherwise the steps stand in for them (ensure_acol).
+    const bool write_acol = !c->uorder_cells || c->p.solver == VOX_SOLVER_LEARNED_FCG;
+    a.acol = write_acol ? c->acol : nullptr; a.rowlen = c->rowlen; a.aval = c->aval;
     if (!c->dcol) VOX_HIP(hipMalloc(&c->dcol, sizeof(uint16_t) * (size_t)sl_size(n, c->a_cap)));
     if (!c->cbase) VOX_HIP(hipMalloc(&c->cbase, sizeof(int32_t) * (size_t)n));
     a.dcol = reinterpret_cast<v4u*>(c->dcol); a.cbase = c->cbase; a.dbad = c->dflags + FLAG_DCOL_BAD;
@@ -475,18 +492,21 @@ int ppe_assemble(vox_ctx* c)
     k_pack_asm<<<grid_for(n), kBlock, 0, c->stream>>>(a);
     // rows: a grid for every particle, of which the blocks past nu (read on
     // the device) return at once
-    switch (kt_of(p.ktype) * 2 + (fp32_only ? 1 : 0)) {
-    case 4: k_assemble_rows<double, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    case 5: k_assemble_rows<float, 2><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    case 12: k_assemble_rows<double, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    case 13: k_assemble_rows<float, 6><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    case 1: k_assemble_rows<float, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    default: k_assemble_rows<double, 0><<<grid_for(n), kBlock, 0, c->stream>>>(a); break;
-    }
+    launch_assemble_rows(c, a, fp32_only);
     VOX_HIP(hipGetLastError());
     VOX_TRY(read_flags(c));
     c->nu = c->hflags[FLAG_NU];
     c->dcol_ok = !c->hflags[FLAG_DCOL_BAD];
+    c->acol_ok = write_acol;
+    if (!c->dcol_ok && !write_acol) {
+        // a step did not fit: the rows again, with their 32-bit columns (the
+        // same values; the kernel is deterministic)
+        a.acol = c->acol;
+        a.dcol = nullptr;
+        launch_assemble_rows(c, a, fp32_only);
+        VOX_HIP(hipGetLastError());
+        c->acol_ok = true;
+    }
     c->col_below = c->hflags[FLAG_COL_BELOW];
     c->col_above = c->hflags[FLAG_COL_ABOVE];
     c->asm_count += 1;
@@ -570,12 +590,47 @@ int ppe_make_aval32(vox_ctx* c)
     return VOX_OK;
 }
 
+// The 32-bit columns from the 16-bit steps, one thread per row.
+__global__ void k_acol_from_steps(int nu, int cap, const int32_t* __restrict__ rowlen,
+                                  const v4u* __restrict__ dcol, const int32_t* __restrict__ cbase,
+                                  int32_t* __restrict__ acol)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nu) return;
+    const int len = rowlen[r];
+    const v4u* __restrict__ dc = dcol + dc_index(r, 0, cap);
+    int32_t* __restrict__ col = acol + mx_index(r, 0, cap);
+    int cj = cbase[r];
+    for (int k = 0; k < len; k += 8) {
+        const v4u w = dc[8L * k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (k + u < len) {
+                cj += (int)((w[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+                col[64L * (k + u)] = cj;
+            }
+        }
+    }
+}
+
+int ensure_acol(vox_ctx* c)
+{
+    if (c->acol_ok || c->nu <= 0) return VOX_OK;
+    if (!c->dcol_ok) return fail(VOX_ERR_STATE, "PPE columns unavailable");
+    k_acol_from_steps<<<grid_for(c->nu), kBlock, 0, c->stream>>>(
+        c->nu, c->a_cap, c->rowlen, reinterpret_cast<const v4u*>(c->dcol), c->cbase, c->acol);
+    VOX_HIP(hipGetLastError());
+    c->acol_ok = true;
+    return VOX_OK;
+}
+
 // CSR over the unknowns: the diagonal first, then the stored entries in list
 // order.
 int ppe_export(vox_ctx* c, int32_t* nu_out, int32_t* nnz_out, int32_t* rowptr, int32_t* cols,
                double* vals, double* rhs, int32_t* ids)
 {
     const int nu = c->nu;
+    VOX_TRY(ensure_acol(c));
     std::vector<int32_t> len(nu > 0 ? nu : 1);
     if (nu > 0)
         VOX_HIP(hipMemcpy(len.data(), c->rowlen, sizeof(int32_t) * nu, hipMemcpyDeviceToHost));
@@ -700,6 +755,7 @@ int ppe_load(vox_ctx* c, int32_t nu, int32_t nnz, const int32_t* rowptr, const i
     c->aval32_ok = false;
     c->aval64_ok = true;
     c->dcol_ok = false;            // the product reads the 32-bit columns
+    c->acol_ok = true;
     c->asm_count += 1;
     c->uorder_cells = false;       // the rows keep the loaded order
     return VOX_OK;
