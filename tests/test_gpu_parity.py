@@ -566,6 +566,38 @@ def test_row_capacity_grows(hip, oracle, dim, mode, caps):
     c.close()
 
 
+def test_ppe_columns_past_16_bit_steps(hip, oracle):
+    """A 2D water column 17,000 particles tall: one cell column of list L's grid
+    holds more than 65,535 unknowns, so the step from a row's last column in
+    one x-column of cells to its first in the next does not fit 16 bits. The
+    assembly then stores the 32-bit columns (a second pass), the product
+    reads them, and the system is still the oracle's, bit for bit."""
+    case = vx.dam_break(2, 0.01, (8, 17000), (9, 17000))
+    p = vx.dam2d_params(case)
+    p.cg_maxit = 20
+    g, c = _pair(case, p, hip, oracle)
+    for sim in (g, c):
+        sim.init()
+        random_state(sim, seed=11)
+    _sync_fields(c, g, ["U", "V", "W", "UNEW", "VNEW", "WNEW", "PNEW", "PHAT", "P"])
+    for sim in (g, c):
+        for st in ("PNUM", "NEIGHBOR", "V_FRACTION", "VISCOSITY", "PREDICTION", "PNUMSTAR",
+                   "BCON", "MATRIX"):
+            getattr(sim, st)()
+    Ag, Ac = g.ppe(), c.ppe()
+    assert Ag["rhs"].shape[0] > 2 * 65536 and np.count_nonzero(Ag["rhs"]) > 0
+    for k in ("rowptr", "cols", "vals", "rhs", "ids"):
+        assert np.array_equal(Ag[k], Ac[k]), f"tall column MATRIX {k}"
+    try:
+        g.CGM()
+    except vx.VoxError:
+        pass                                             # 20 iterations need not converge
+    st = g.status()
+    assert st.product_col_bytes == 4 and st.product_window_cols == 0
+    g.close()
+    c.close()
+
+
 def test_domain_error_is_reported(hip):
     case = vx.dam_break_2d_c0()
     p = vx.dam2d_params(case)

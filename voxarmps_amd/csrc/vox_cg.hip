@@ -253,6 +253,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_win(CgArgs<VT, MT> a)
     double pq = 0.0;
     const int r = ch * kBlock + threadIdx.x;
     if (r < a.nu) {
+        // (issuing the first batch before the staging, and each batch before
+        // the previous one is consumed, measured the same)
         const VT pr = a.p[r];
         VT acc = (VT)a.diag[r] * pr;
         const int len = a.rowlen[r];
