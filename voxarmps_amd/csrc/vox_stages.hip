@@ -668,13 +668,15 @@ __global__ __launch_bounds__(BS) void k_prediction_dg(PredArgs a)
             gq[nq * BS] = jj[u];
             ++nq;
         }
-        const double wl = wfun_r(D, a.ktype, a.re_l, inv_re_l);
-        s4 = use ? s4 + wl * du * NEU : s4;
-        s5 = use ? s5 + wl * dv * NEU : s5;
-        s7 = use ? s7 + wl * du : s7;
-        s8 = use ? s8 + wl * dv : s8;
-        s6 = use ? s6 + wl * dw * NEU : s6;
-        s9 = use ? s9 + wl * dw : s9;
+        // a skipped entry adds terms with W = +0: +-0 for finite fields,
+        // which leave each sum's bits unchanged (one select instead of six)
+        const double wl = use ? wfun_r(D, a.ktype, a.re_l, inv_re_l) : 0.0;
+        s4 = s4 + wl * du * NEU;
+        s5 = s5 + wl * dv * NEU;
+        s7 = s7 + wl * du;
+        s8 = s8 + wl * dv;
+        s6 = s6 + wl * dw * NEU;
+        s9 = s9 + wl * dw;
     }
     }
     }
