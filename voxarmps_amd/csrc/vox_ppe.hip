@@ -145,9 +145,10 @@ struct AsmArgs {
 // (functions.cu:990-1001), 0 for every particle that is not an unknown.
 __global__ void k_pack_asm(AsmArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
-    const int i = a.order ? a.order[t].id : t;
+    // threads in particle order (fields read in lines), whole records stored
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int t = a.order ? a.pos_of[i] : i;
     PosRec r;
     r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
     const int bi = a.bcon[i];

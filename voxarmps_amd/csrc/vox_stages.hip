@@ -426,6 +426,7 @@ int st_viscosity(vox_ctx* c)
 // ---- PREDICTION: functions.cu:2283-2369 ------------------------------------------------------------
 struct PredArgs {
     int n, dim, ktype;
+    const int32_t* pos_of;    // pos: each particle's position in order
     double re_s, re_l, rho1, rho2, gx, gy, gz, dt, lapc, ln0, gcoef_num, n0s, relax1;
     ListView L;
     const int32_t* pt;
@@ -441,12 +442,13 @@ struct PredArgs {
 
 // One 64-byte record per particle with every field the predictor gathers, at
 // the particle's id, or at its position in the cell-sorted order (with its
-// type beside it) when the rows hold positions.
+// type beside it) when the rows hold positions. Threads run in particle
+// order (the fields are read in lines) and store whole records.
 __global__ void k_pack_pred(PredArgs a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
-    const int i = a.pos ? a.order[t].id : t;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int t = a.pos ? a.pos_of[i] : i;
     if (a.pos) a.rpt[t] = a.pt[i];
     PredRec r;
     r.x = a.x[i]; r.y = a.y[i]; r.z = (a.dim == 3) ? a.z[i] : 0.0;
@@ -781,6 +783,7 @@ int st_prediction(vox_ctx* c)
     // (2.5% faster than particle order, whose row reads are contiguous)
     a.order = list_order(c, VOX_LIST_L);
     a.pos = list_of(c, VOX_LIST_L).pos_rows && a.order;
+    a.pos_of = list_of(c, VOX_LIST_L).pos_of;
     a.rpt = c->pt_sorted;
     k_pack_pred<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     VOX_TRY(launch_prediction(c, a, c->tune_pred));

@@ -89,13 +89,15 @@ struct Pred32Args {
     bool pos;                 // row entries are positions in order (list L's compact
                               // rows, stored at positions): rec and rpt are in that
                               // order and thread t's own record is rec[t]
+    const int32_t* pos_of;    // pos: each particle's position in order
 };
 
+// Threads in particle order (fields read in lines); whole records stored.
 __global__ void k_pack_pred32(Pred32Args a)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
-    const int i = a.pos ? a.order[t].id : t;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int t = a.pos ? a.pos_of[i] : i;
     if (a.pos) a.rpt[t] = a.pt[i];
     Pred32 r;
     r.x = qpos(a.x[i], a.q.ox, a.q.ix);
@@ -212,6 +214,7 @@ int st_prediction32(vox_ctx* c)
     // threads in list L's cell order; rows of positions index records in that order
     a.order = list_order(c, VOX_LIST_L);
     a.pos = list_of(c, VOX_LIST_L).pos_rows && a.order;
+    a.pos_of = list_of(c, VOX_LIST_L).pos_of;
     a.rpt = c->pt_sorted;
     k_pack_pred32<<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
     if (uniform_meu(c)) k_prediction32<true><<<grid_for(c->tp), kBlock, 0, c->stream>>>(a);
