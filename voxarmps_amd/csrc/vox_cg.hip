@@ -39,7 +39,37 @@
 
 namespace vox {
 
-constexpr int kCgBlocks = 2048;   // 8192 waves: 8 per SIMD on 256 CUs
+// The vector kernels (everything but the products and the preconditioner):
+// 512 blocks of 1024 threads, 8192 waves (8 per SIMD on 256 CUs). Against
+// 2048 blocks of 256 threads, each kernel's blocks read the previous
+// kernel's partials a quarter as often, and there are a quarter as many.
+constexpr int kVecThreads = 1024;
+constexpr int kVecBlocks = 512;
+
+// Fixed-order sum over a block of NT threads (LDS tree of NT doubles).
+template <int NT>
+__device__ __forceinline__ double block_sum_n(double v, double* lds)
+{
+    const int t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) lds[t] = lds[t] + lds[t + s];
+        __syncthreads();
+    }
+    const double r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+template <int NT>
+__device__ __forceinline__ double sum_partials_n(const double* part, int n, double* lds)
+{
+    double s = 0.0;
+    for (int k = threadIdx.x; k < n; k += NT) s = s + part[k];
+    return block_sum_n<NT>(s, lds);
+}
 
 // partial slots (kMaxPartials doubles each)
 enum { P_BB = 0, P_RZ0, P_NNZ, P_PQ, P_RR, P_RZ, P_RES, P_RHS, P_ZQ, P_COUNT };
@@ -89,9 +119,9 @@ __device__ __forceinline__ double sum_partials(const double* part, int n, double
 }
 
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_cg_init(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     double bb = 0.0, rz = 0.0, nz = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
         const double b = a.rhs[r];
@@ -104,9 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(CgArgs<VT, MT> a)
         rz += (double)rv * (double)z;
         nz += (double)(a.rowlen[r] + 1);
     }
-    bb = block_sum(bb, lds);
-    rz = block_sum(rz, lds);
-    nz = block_sum(nz, lds);
+    bb = block_sum_n<kVecThreads>(bb, lds);
+    rz = block_sum_n<kVecThreads>(rz, lds);
+    nz = block_sum_n<kVecThreads>(nz, lds);
     if (threadIdx.x == 0) {
         PSLOT(a, P_BB)[blockIdx.x] = bb;
         PSLOT(a, P_RZ0)[blockIdx.x] = rz;
@@ -118,9 +148,9 @@ template <class VT, class MT>
 __global__ __launch_bounds__(kBlock) void k_cg_init_fin(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
-    double bb = sum_partials(PSLOT(a, P_BB), kCgBlocks, lds);
-    double rz = sum_partials(PSLOT(a, P_RZ0), kCgBlocks, lds);
-    double nz = sum_partials(PSLOT(a, P_NNZ), kCgBlocks, lds);
+    double bb = sum_partials(PSLOT(a, P_BB), kVecBlocks, lds);
+    double rz = sum_partials(PSLOT(a, P_RZ0), kVecBlocks, lds);
+    double nz = sum_partials(PSLOT(a, P_NNZ), kVecBlocks, lds);
     if (threadIdx.x == 0) {
         double bn = sqrt(bb);
         a.scal[S_BNORM] = bn;
@@ -437,11 +467,11 @@ __global__ __launch_bounds__(kBlock) void k_win_build(WinArgs a)
 }
 
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_cg_update(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     if (a.flags[FLAG_CG_DONE]) return;
-    const double pq = sum_partials(PSLOT(a, P_PQ), a.npq, lds);
+    const double pq = sum_partials_n<kVecThreads>(PSLOT(a, P_PQ), a.npq, lds);
     const double rz_old = a.scal[S_RZ0 + ((a.it - 1) & 1)];
     if (!(pq > 0.0) || !isfinite(pq)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -462,8 +492,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<VT, MT> a)
         rr += (double)res * (double)res;
         rz += (double)res * (double)z;
     }
-    rr = block_sum(rr, lds);
-    rz = block_sum(rz, lds);
+    rr = block_sum_n<kVecThreads>(rr, lds);
+    rz = block_sum_n<kVecThreads>(rz, lds);
     if (threadIdx.x == 0) {
         PSLOT(a, P_RR)[blockIdx.x] = rr;
         PSLOT(a, P_RZ)[blockIdx.x] = rz;
@@ -471,12 +501,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<VT, MT> a)
 }
 
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_cg_p(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     if (a.flags[FLAG_CG_DONE]) return;
-    const double rr = sum_partials(PSLOT(a, P_RR), kCgBlocks, lds);
-    const double rz = sum_partials(PSLOT(a, P_RZ), kCgBlocks, lds);
+    const double rr = sum_partials_n<kVecThreads>(PSLOT(a, P_RR), kVecBlocks, lds);
+    const double rz = sum_partials_n<kVecThreads>(PSLOT(a, P_RZ), kVecBlocks, lds);
     const double rel = sqrt(rr) / a.scal[S_BNORM];
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     if (lead) {
@@ -499,9 +529,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<VT, MT> a)
 
 // ---- flexible CG ---------------------------------------------------------------------------
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_fcg_init(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_fcg_init(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     double bb = 0.0, nz = 0.0;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < a.nu; r += gridDim.x * blockDim.x) {
         const double b = a.rhs[r];
@@ -510,8 +540,8 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init(CgArgs<VT, MT> a)
         bb += b * b;
         nz += (double)(a.rowlen[r] + 1);
     }
-    bb = block_sum(bb, lds);
-    nz = block_sum(nz, lds);
+    bb = block_sum_n<kVecThreads>(bb, lds);
+    nz = block_sum_n<kVecThreads>(nz, lds);
     if (threadIdx.x == 0) {
         PSLOT(a, P_BB)[blockIdx.x] = bb;
         PSLOT(a, P_NNZ)[blockIdx.x] = nz;
@@ -566,8 +596,8 @@ template <class VT, class MT>
 __global__ __launch_bounds__(kBlock) void k_fcg_init_fin(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
-    const double bb = sum_partials(PSLOT(a, P_BB), kCgBlocks, lds);
-    const double nz = sum_partials(PSLOT(a, P_NNZ), kCgBlocks, lds);
+    const double bb = sum_partials(PSLOT(a, P_BB), kVecBlocks, lds);
+    const double nz = sum_partials(PSLOT(a, P_NNZ), kVecBlocks, lds);
     const double rz = sum_partials(PSLOT(a, P_RZ0), a.npq, lds);
     if (threadIdx.x == 0) {
         const double bn = sqrt(bb);
@@ -585,11 +615,11 @@ __global__ __launch_bounds__(kBlock) void k_fcg_init_fin(CgArgs<VT, MT> a)
 }
 
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_fcg_update(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_fcg_update(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     if (a.flags[FLAG_CG_DONE]) return;
-    const double pq = sum_partials(PSLOT(a, P_PQ), a.npq, lds);
+    const double pq = sum_partials_n<kVecThreads>(PSLOT(a, P_PQ), a.npq, lds);
     const double rz_old = a.scal[S_RZ0 + ((a.it - 1) & 1)];
     if (!(pq > 0.0) || !isfinite(pq)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -607,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(CgArgs<VT, MT> a)
         a.r[r] = res;
         rr += (double)res * (double)res;
     }
-    rr = block_sum(rr, lds);
+    rr = block_sum_n<kVecThreads>(rr, lds);
     if (threadIdx.x == 0) PSLOT(a, P_RR)[blockIdx.x] = rr;
 }
 
@@ -616,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_fcg_prec(CgArgs<VT, MT> a)
 {
     __shared__ double lds[kBlock];
     if (a.flags[FLAG_CG_DONE]) return;
-    const double rr = sum_partials(PSLOT(a, P_RR), kCgBlocks, lds);
+    const double rr = sum_partials(PSLOT(a, P_RR), kVecBlocks, lds);
     const double rel = sqrt(rr) / a.scal[S_BNORM];
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     if (lead) {
@@ -648,13 +678,13 @@ __global__ __launch_bounds__(kBlock) void k_fcg_prec(CgArgs<VT, MT> a)
 }
 
 template <class VT, class MT>
-__global__ __launch_bounds__(kBlock) void k_fcg_p(CgArgs<VT, MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_fcg_p(CgArgs<VT, MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     if (a.flags[FLAG_CG_DONE]) return;
-    const double pq = sum_partials(PSLOT(a, P_PQ), a.npq, lds);
-    const double zr = sum_partials(PSLOT(a, P_RZ), a.npq, lds);
-    const double zq = sum_partials(PSLOT(a, P_ZQ), a.npq, lds);
+    const double pq = sum_partials_n<kVecThreads>(PSLOT(a, P_PQ), a.npq, lds);
+    const double zr = sum_partials_n<kVecThreads>(PSLOT(a, P_RZ), a.npq, lds);
+    const double zq = sum_partials_n<kVecThreads>(PSLOT(a, P_ZQ), a.npq, lds);
     const double rz_old = a.scal[S_RZ0 + ((a.it - 1) & 1)];
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     if (!(zr > 0.0) || !isfinite(zr)) {
@@ -691,9 +721,9 @@ struct IrArgs {
 // FP32 after a mixed-precision assembly); partials ||res||^2 and ||b||^2 per
 // block.
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs<MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_ir_residual(IrArgs<MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     double rr = 0.0, bb = 0.0;
     const int r = xcd_tid();
     if (r < a.nu) {
@@ -733,11 +763,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(IrArgs<MT> a)
 
 // The same residual with x gathered from the product's LDS windows.
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_ir_residual_win(IrArgs<MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_ir_residual_win(IrArgs<MT> a)
 {
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     double* win = reinterpret_cast<double*>(dyn_lds);
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     const int ch = xcd_block();
     const int n = a.wn[ch];
     const int32_t* __restrict__ uq = a.wuniq + (long)ch * kWinMax;
@@ -780,9 +810,9 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual_win(IrArgs<MT> a)
 }
 
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_ir_residual_fin(IrArgs<MT> a)
+__global__ __launch_bounds__(kVecThreads) void k_ir_residual_fin(IrArgs<MT> a)
 {
-    __shared__ double lds[kBlock];
+    __shared__ double lds[kVecThreads];
     const double rr = sum_partials(PSLOT(a, P_RES), a.npq, lds);
     const double bb = sum_partials(PSLOT(a, P_RHS), a.npq, lds);
     if (threadIdx.x == 0) {
@@ -849,7 +879,7 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
 {
     {
         ProfScope ps(c, KC_CG_INIT);
-        k_cg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+        k_cg_init<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
         k_cg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
         VOX_HIP(hipGetLastError());
     }
@@ -874,11 +904,11 @@ static int run_cg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
             }
             {
                 ProfScope ps(c, KC_CG_UPDATE);
-                k_cg_update<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_cg_update<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
             }
             {
                 ProfScope ps(c, KC_CG_P);
-                k_cg_p<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_cg_p<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
             }
         }
         VOX_HIP(hipGetLastError());
@@ -898,7 +928,7 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
 {
     {
         ProfScope ps(c, KC_CG_INIT);
-        k_fcg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+        k_fcg_init<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
     }
     {
         ProfScope ps(c, KC_LPC_APPLY);
@@ -925,7 +955,7 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
             }
             {
                 ProfScope ps(c, KC_CG_UPDATE);
-                k_fcg_update<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_fcg_update<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
             }
             {
                 ProfScope ps(c, KC_LPC_APPLY);
@@ -933,7 +963,7 @@ static int run_fcg(vox_ctx* c, CgArgs<VT, MT>& a, int* iters_out)
             }
             {
                 ProfScope ps(c, KC_CG_P);
-                k_fcg_p<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+                k_fcg_p<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
             }
         }
         VOX_HIP(hipGetLastError());
@@ -1206,7 +1236,7 @@ static int solve_learned(vox_ctx* c, vox_solve_info* info)
 template <class VT, class MT>
 static int time_spmv_t(vox_ctx* c, CgArgs<VT, MT>& a, int reps, float* ms)
 {
-    k_cg_init<VT, MT><<<kCgBlocks, kBlock, 0, c->stream>>>(a);
+    k_cg_init<VT, MT><<<kVecBlocks, kVecThreads, 0, c->stream>>>(a);
     k_cg_init_fin<VT, MT><<<1, kBlock, 0, c->stream>>>(a);
     VOX_HIP(hipGetLastError());
     hipEvent_t e0, e1;
