@@ -22,6 +22,11 @@
 // FP32 matrix values and FP32 vectors (FP64 dots) to a relative eta.
 //
 // The matrix is SELL-64 (mx_index); the product runs one thread per row.
+// Its default form (k_cg_spmv_win) stages p at each 256-row block's distinct
+// columns in LDS (the windows k_win_build lists once per assembly) and reads
+// each entry's 16-bit rank in that window instead of its column; the other
+// forms read the columns as 16-bit steps (dcol) or 32-bit ids (acol). Every
+// form sums each row in the same order: the same q, bit for bit.
 //
 // Flexible CG (solver VOX_SOLVER_LEARNED_FCG) takes z = M r from the learned
 // coupling operator (vox_lpc.hip) instead of r / d, and one more kernel per
