@@ -81,6 +81,44 @@ def test_ensemble_matches_solo_runs(hip):
 
 
 @pytest.mark.gpu
+def test_ensemble_scenario_fp64_against_oracle(hip, oracle):
+    """One scenario of the ensemble configuration run alone in FP64 on the GPU
+    and on the oracle: two fluids, SPS-LES on, Herschel-Bulkley (Bingham)
+    fluid 2, two radii, PPE to 1e-10, 30 steps. Lists stay identical, and
+    positions, velocities and pressure agree to 1e-6 of their scale (the
+    solves agree to their tolerance; the Herschel-Bulkley viscosity's libm pow
+    may differ in the last ulp)."""
+    from voxarmps_amd._abi import PREC_FP64
+    case, region, boxes, scen = _small_ensemble()
+    s = scen[1]
+    p = vx.ensemble_params(s, tol=1e-10, box=boxes[1])
+    p.precision = PREC_FP64
+    g = vx.Simulation(p, s, backend=hip)
+    c = vx.Simulation(p, s, backend=oracle)
+    g.init()
+    c.init()
+    assert g.n0() == c.n0()
+    g.step(30)
+    c.step(30)
+    f = s.ptype > 0
+    assert (s.ptype == 2).any()
+    for n in ("X", "Y", "U", "V", "P", "MEU"):
+        a, b = g.get(n)[f], c.get(n)[f]
+        scale = max(np.max(np.abs(b)), 1.0)
+        err = np.max(np.abs(a - b))
+        print(f"{n}: max |gpu - oracle| {err:.2e} (scale {scale:.2e})")
+        assert err <= 1e-6 * scale, n
+    for w in (vx.LIST_S, vx.LIST_L):
+        cg, ig = g.neighbors(w)
+        cc, ic = c.neighbors(w)
+        assert np.array_equal(cg, cc) and np.array_equal(ig, ic), f"list {w}"
+    sg, sc = g.status(), c.status()
+    assert sg.nan_count == sc.nan_count == 0
+    g.close()
+    c.close()
+
+
+@pytest.mark.gpu
 def test_autotune_keeps_lists(hip):
     """The autotuner only picks the faster search walk: the lists before and
     after it are identical."""

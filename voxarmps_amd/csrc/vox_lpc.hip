@@ -24,7 +24,7 @@
 // state, 16 bf16 = 32 bytes of message per row), so the aggregation gathers
 // one 32-byte message per matrix entry; matrix and coupling operator are
 // SELL-64 (mx_index). The dense layers run one wavefront per 16-row tile:
-// v_mfma_f32_16x16x16_bf16 with the weights held in registers for the whole
+// v_mfma_f32_16x16x32_bf16 with the weights held in registers for the whole
 // launch, the 16 x 64 output tile staged in LDS for the coalesced store and
 // for the message product that follows in the same kernel.
 //
@@ -53,8 +53,6 @@ constexpr float kBeta = -0.40546510810816438f;   // -ln(1.5): c = 1 at zero outp
 constexpr int kProbes = 4;
 constexpr int kOuterBlocks = 256;
 constexpr int kOuterMax = LH * LH;
-
-typedef short v4s __attribute__((ext_vector_type(4)));
 
 // ---- bf16 (round to nearest even; inputs are finite) -----------------------
 __device__ __forceinline__ uint16_t f2bf(float x)
@@ -295,12 +293,25 @@ struct DenseArgs {
     float* o;                   // network output (layer 3)
 };
 
-__device__ __forceinline__ v4s ld4(const uint16_t* p) { return *reinterpret_cast<const v4s*>(p); }
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-// One 16-row tile per wavefront. Operand layouts of v_mfma_f32_16x16x16_bf16:
-// lane l supplies A[row l%16][k 4*(l/16)..+3] and B[k 4*(l/16)..+3][col l%16]
+__device__ __forceinline__ v8s ld8(const uint16_t* p) { return *reinterpret_cast<const v8s*>(p); }
+
+// D += A B on gfx950's 16 x 16 x 32 bf16 matrix instruction.
+__device__ __forceinline__ v4f mfma32(v8s x, v8s y, v4f acc)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, x),
+                                                   __builtin_bit_cast(v8bf, y), acc, 0, 0, 0);
+}
+
+// One 16-row tile per wavefront. Operand layouts of v_mfma_f32_16x16x32_bf16:
+// lane l supplies A[row l%16][k 8*(l/16)..+7] and B[k 8*(l/16)..+7][col l%16]
 // and receives D[row 4*(l/16)+i][col l%16], i = 0..3. Rows are graph rows,
-// columns are output features, k runs over input features.
+// columns are output features, k runs over input features: the encoder's 8
+// features are one instruction (k 8..31 zero), a message layer's 64 hidden
+// inputs two and its 16 messages one (k 16..31 zero), the message
+// projection's 64 inputs two.
 template <int MODE>   // 0: encoder (features in), 1: message layer
 __global__ __launch_bounds__(kBlock) void k_lpc_dense(DenseArgs a)
 {
@@ -308,28 +319,29 @@ __global__ __launch_bounds__(kBlock) void k_lpc_dense(DenseArgs a)
     __shared__ __align__(16) uint16_t ts[4][16][LM];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int cl = lane & 15, gp = lane >> 4;
-    constexpr int KB = (MODE == 0) ? 1 : 5;
+    constexpr int KB = (MODE == 0) ? 1 : 3;
+    const v8s zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
     const int l = a.layer;
-    v4s bw[KB][4];
+    v8s bw[KB][4];
     float bias[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const int o = nt * 16 + cl;
         if (MODE == 0) {
-            bw[0][nt] = (gp < 2) ? ld4(a.w16 + OFF_W0 + o * LF + 4 * gp) : v4s{0, 0, 0, 0};
+            bw[0][nt] = (gp == 0) ? ld8(a.w16 + OFF_W0 + o * LF) : zero8;
             bias[nt] = a.w32[OFF_B0 + o];
         } else {
 #pragma unroll
-            for (int kb = 0; kb < 4; ++kb) bw[kb][nt] = ld4(a.w16 + off_U(l) + o * LH + kb * 16 + 4 * gp);
-            bw[KB - 1][nt] = ld4(a.w16 + off_V(l) + o * LM + 4 * gp);
+            for (int kb = 0; kb < 2; ++kb) bw[kb][nt] = ld8(a.w16 + off_U(l) + o * LH + kb * 32 + 8 * gp);
+            bw[KB - 1][nt] = (gp < 2) ? ld8(a.w16 + off_V(l) + o * LM + 8 * gp) : zero8;
             bias[nt] = a.w32[off_c(l) + o];
         }
     }
     const bool next = (l < LNL);
-    v4s bp[4];
+    v8s bp[2];
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
-        bp[kb] = next ? ld4(a.w16 + off_P(l + 1) + cl * LH + kb * 16 + 4 * gp) : v4s{0, 0, 0, 0};
+    for (int kb = 0; kb < 2; ++kb)
+        bp[kb] = next ? ld8(a.w16 + off_P(l + 1) + cl * LH + kb * 32 + 8 * gp) : zero8;
     float wo[2][LH / 2];   // head weights of this lane's output (lanes < 32)
     if (!next) {
         const int q = lane & 1;
@@ -345,26 +357,27 @@ __global__ __launch_bounds__(kBlock) void k_lpc_dense(DenseArgs a)
         const bool act = tile < ntile;
         const long row0 = (long)tile * 16;
         if (act) {
-            v4s af[KB];
+            v8s af[KB];
             if (MODE == 0) {
-                if (gp < 2) {
-                    const float* f = a.feat + (row0 + cl) * LF + 4 * gp;
-                    af[0] = v4s{(short)f2bf(f[0]), (short)f2bf(f[1]), (short)f2bf(f[2]),
-                                (short)f2bf(f[3])};
+                if (gp == 0) {
+                    const float* f = a.feat + (row0 + cl) * LF;
+                    af[0] = v8s{(short)f2bf(f[0]), (short)f2bf(f[1]), (short)f2bf(f[2]),
+                                (short)f2bf(f[3]), (short)f2bf(f[4]), (short)f2bf(f[5]),
+                                (short)f2bf(f[6]), (short)f2bf(f[7])};
                 } else {
-                    af[0] = v4s{0, 0, 0, 0};
+                    af[0] = zero8;
                 }
             } else {
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb) af[kb] = ld4(a.hin + (row0 + cl) * LH + kb * 16 + 4 * gp);
-                af[KB - 1] = ld4(a.min + (row0 + cl) * LM + 4 * gp);
+                for (int kb = 0; kb < 2; ++kb) af[kb] = ld8(a.hin + (row0 + cl) * LH + kb * 32 + 8 * gp);
+                if (gp < 2) af[KB - 1] = ld8(a.min + (row0 + cl) * LM + 8 * gp);
+                else af[KB - 1] = zero8;
             }
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt) {
                 v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[kb], bw[kb][nt], acc, 0, 0, 0);
+                for (int kb = 0; kb < KB; ++kb) acc = mfma32(af[kb], bw[kb][nt], acc);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     th[wv][4 * gp + i][nt * 16 + cl] = f2bf(fmaxf(acc[i] + bias[nt], 0.f));
@@ -380,9 +393,9 @@ __global__ __launch_bounds__(kBlock) void k_lpc_dense(DenseArgs a)
             if (next) {
                 v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kb = 0; kb < 4; ++kb) {
-                    const v4s x = *reinterpret_cast<const v4s*>(&th[wv][cl][kb * 16 + 4 * gp]);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(x, bp[kb], acc, 0, 0, 0);
+                for (int kb = 0; kb < 2; ++kb) {
+                    const v8s x = *reinterpret_cast<const v8s*>(&th[wv][cl][kb * 32 + 8 * gp]);
+                    acc = mfma32(x, bp[kb], acc);
                 }
 #pragma unroll
                 for (int i = 0; i < 4; ++i) ts[wv][4 * gp + i][cl] = f2bf(acc[i]);
