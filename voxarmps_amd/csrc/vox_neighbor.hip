@@ -85,11 +85,32 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* lds, int3
     return incl - v;
 }
 
-__global__ void k_scan_tiles(const int32_t* __restrict__ in, int32_t* __restrict__ out,
-                             int32_t* __restrict__ block_sums, long n)
+// Exclusive scan in two launches: the sum of each tile of kScanTile items,
+// then per tile its offset (the sum of the tiles before it, which each block
+// adds up from the first pass's sums) plus the tile's own exclusive scan.
+__global__ void k_scan_tile_sums(const int32_t* __restrict__ in, int32_t* __restrict__ block_sums,
+                                 long n)
 {
     __shared__ int32_t lds[kBlock];
-    long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+    const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += (base + k < n) ? in[base + k] : 0;
+    int32_t tot;
+    (void)block_excl_scan(s, lds, &tot);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// total (if given) receives the sum of all n items; it may alias out + n.
+__global__ void k_scan_out(const int32_t* in, int32_t* out, const int32_t* __restrict__ block_sums,
+                           long n, int32_t* total)
+{
+    __shared__ int32_t lds[kBlock];
+    int32_t before = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += kBlock) before += block_sums[k];
+    int32_t off;
+    (void)block_excl_scan(before, lds, &off);
+    const long base = (long)blockIdx.x * kScanTile + (long)threadIdx.x * kScanItems;
     int32_t v[kScanItems];
     int32_t s = 0;
 #pragma unroll
@@ -98,40 +119,13 @@ __global__ void k_scan_tiles(const int32_t* __restrict__ in, int32_t* __restrict
         s += v[k];
     }
     int32_t tot;
-    int32_t ex = block_excl_scan(s, lds, &tot);
+    int32_t ex = off + block_excl_scan(s, lds, &tot);
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < n) out[base + k] = ex;
         ex += v[k];
     }
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
-}
-
-// single block: scan of the block sums (any count, looped)
-__global__ void k_scan_sums(int32_t* __restrict__ sums, long nb, int32_t* __restrict__ total)
-{
-    __shared__ int32_t lds[kBlock];
-    int32_t carry = 0;
-    for (long base = 0; base < nb; base += kBlock) {
-        long i = base + threadIdx.x;
-        int32_t v = (i < nb) ? sums[i] : 0;
-        int32_t tot;
-        int32_t ex = block_excl_scan(v, lds, &tot);
-        if (i < nb) sums[i] = ex + carry;
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
-}
-
-__global__ void k_scan_add(int32_t* __restrict__ out, const int32_t* __restrict__ sums, long n)
-{
-    long base = (long)blockIdx.x * kScanTile;
-    int32_t add = sums[blockIdx.x];
-    for (int k = threadIdx.x; k < kScanTile; k += kBlock) {
-        long i = base + k;
-        if (i < n) out[i] += add;
-    }
+    if (total && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *total = off + tot;
 }
 
 // out[0..n) = exclusive scan of in[0..n); out[n] = total if out has room (caller
@@ -146,9 +140,8 @@ int scan_exclusive(vox_ctx* c, const int32_t* in, int32_t* out, long n, int32_t*
         VOX_HIP(hipMalloc(&c->scan_blocks, sizeof(int32_t) * (nb + 64)));
         c->scan_blocks_cap = nb + 64;
     }
-    k_scan_tiles<<<(int)nb, kBlock, 0, c->stream>>>(in, out, c->scan_blocks, n);
-    k_scan_sums<<<1, kBlock, 0, c->stream>>>(c->scan_blocks, nb, total_dev);
-    if (nb > 1) k_scan_add<<<(int)nb, kBlock, 0, c->stream>>>(out, c->scan_blocks, n);
+    k_scan_tile_sums<<<(int)nb, kBlock, 0, c->stream>>>(in, c->scan_blocks, n);
+    k_scan_out<<<(int)nb, kBlock, 0, c->stream>>>(in, out, c->scan_blocks, n, total_dev);
     VOX_HIP(hipGetLastError());
     return VOX_OK;
 }
