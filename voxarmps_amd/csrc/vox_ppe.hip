@@ -196,6 +196,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
     bool bad = false;
     const int cnt = live ? a.L.cnt[self] : 0;
     constexpr int U = 8;
+    // the next chunk's ids are loaded while the current chunk's records are
+    // gathered and used
+    int32_t jn[U];
+    if (cnt > 0) load_chunk8(row, 0, jn);
     for (int g = 0;; ++g) {
         // this lane's rows until group g is complete or the row is done; a
         // list chunk adds at most 8 entries, so the ring never holds more
@@ -203,7 +207,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble_rows(AsmArgs a)
         while (k < 8 * (g + 1) && l0 < cnt) {
             int jj[U];
             PosRec rr[U];
-            load_chunk8(row, l0 >> 3, jj);
+#pragma unroll
+            for (int u = 0; u < U; ++u) jj[u] = jn[u];
+            if (l0 + U < cnt) load_chunk8(row, (l0 >> 3) + 1, jn);
 #pragma unroll
             for (int u = 0; u < U; ++u) jj[u] = (l0 + u < cnt) ? jj[u] : self;
 #pragma unroll
